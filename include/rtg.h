@@ -1,0 +1,266 @@
+/*
+ * rtg.h — C ABI of librtg, the MI355X (gfx950) render path of the RayTrace
+ * per-pixel Whitted renderer.
+ *
+ * The reference has no plugin ABI: its render path sits behind the C++ classes
+ * RayTracer (RayTracer.h:16-56), DrawObject (3DElement.h:185-202) and Scene
+ * (Scene.h:16-48).  This header is the thin, caller-agnostic boundary those
+ * classes are rewired onto (raytrace_amd/host/rt_raytracer.cpp): plain structs,
+ * plain pointers and sizes, int status codes.  Every entry point names the
+ * reference function it replaces.
+ *
+ * Conventions
+ *   - Return value: RTG_OK (0) on success, < 0 on error; rtg_last_error()
+ *     returns a thread-local message for the last failing call on that thread.
+ *   - Host buffers are caller-owned; device memory is library-owned unless an
+ *     entry point says "device pointer".
+ *   - A scene or server handle may be used from one thread at a time.
+ *   - Coordinates follow the reference: output row 0 is the bottom image row
+ *     (ycur = -H/2), 3 bytes per pixel in R,G,B order, row stride 3*W
+ *     (RayTracer.cpp:20-37, 3DElement.cpp:463-468).
+ */
+#ifndef RTG_H
+#define RTG_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RTG_ABI_VERSION 1
+
+/* status codes */
+#define RTG_OK 0
+#define RTG_E_INVALID (-1)   /* bad argument */
+#define RTG_E_DEVICE (-2)    /* HIP runtime error */
+#define RTG_E_NOMEM (-3)     /* device allocation failed / budget exceeded */
+#define RTG_E_CANCELLED (-4) /* rtg_cancel() stopped the job */
+#define RTG_E_STATE (-5)     /* call not valid in the current state */
+#define RTG_E_LIMIT (-6)     /* a documented size limit was exceeded */
+#define RTG_E_PENDING 1      /* rtg_poll: job still running */
+
+/* object types: 3DElement.h:4-8 */
+#define RTG_OBJ_SPHERE 0x1
+#define RTG_OBJ_CUBE 0x2
+#define RTG_OBJ_MODEL 0x3
+#define RTG_OBJ_PLANE 0x4
+#define RTG_OBJ_BALLPLANE 0x5
+
+/* light types: 3DElement.h:13-15 */
+#define RTG_LIGHT_PARALLEL 0x1
+#define RTG_LIGHT_POINT 0x2
+#define RTG_LIGHT_SPOT 0x3
+
+/* render modes: RayTracer.h:5-13 */
+#define RTG_MODE_CHECK 0x1
+#define RTG_MODE_DEPTHTEST 0x2
+#define RTG_MODE_NORMALTEST 0x3
+#define RTG_MODE_TEXTURETEST 0x4
+#define RTG_MODE_MATERIALTEST 0x5
+#define RTG_MODE_SHADOWTEST 0x6
+#define RTG_MODE_REFLECTTEST 0x7
+#define RTG_MODE_REFRACTTEST 0x8
+#define RTG_MODE_RAYTRACE 0x80
+
+/* engines */
+#define RTG_ENGINE_AUTO 0
+#define RTG_ENGINE_MEGAKERNEL 1 /* one launch, one thread per sample, DFS of the ray tree */
+#define RTG_ENGINE_WAVEFRONT 2  /* per-depth extend/shade/occlude kernels over compacted queues */
+
+/* rtg_params.flags */
+#define RTG_FLAG_FP64_RETEST 0x1 /* re-test near-tie primary candidates in FP64 */
+
+/* Hard limits of this build (the reference's own caps are noted beside). */
+#define RTG_MAX_LIGHTS 32      /* reference AddLight caps at 8 (Scene.cpp:85-86) */
+#define RTG_MAX_LEVEL 15       /* reference maxLevel is a uint8 (RayTracer.h:49) */
+#define RTG_MAX_DIM 16384      /* reference output is a fixed 2048x2048 (RayTracer.cpp:603) */
+
+typedef struct rtg_vec4 {
+    float x, y, z, w;
+} rtg_vec4;
+
+/* Material (3DElement.h:106-120).  `tex` indexes rtg_scene_desc.textures. */
+typedef struct rtg_material {
+    rtg_vec4 ambient, diffuse, specular, emission;
+    float shiness, reflect, refract, rfr;
+    int32_t tex; /* -1: no texture (texel colour is white, 3DElement.cpp:432-436) */
+    int32_t pad[3];
+} rtg_material;
+
+/* Texture (3DElement.h:94-104): w*h texels, 3 bytes each in B,G,R order. */
+typedef struct rtg_texture {
+    int32_t w, h;
+    const uint8_t *bgr;
+} rtg_texture;
+
+/* One entry of Scene::Objects (Scene.h:29), flattened. */
+typedef struct rtg_object {
+    int32_t type;  /* RTG_OBJ_* */
+    int32_t show;  /* DrawObject::bShow */
+    int32_t mtl;   /* sphere / plane / cube / ballplane: index into materials */
+    int32_t tex;   /* plane: index into textures (Plane::tex) */
+    rtg_vec4 position;         /* DrawObject::position */
+    float radius, radius_sqr;  /* sphere, ballplane (Basic3DObject.h:8,57) */
+    int32_t part_begin, part_count; /* model: range of rtg_scene_desc.parts */
+    rtg_vec4 normal, axisx, axisy;  /* plane / ballplane frame (Basic3DObject.h:39,43) */
+    rtg_vec4 vmin, vmax; /* model: VerMin/VerMax (model space, Model.h:22); cube: min/max */
+} rtg_object;
+
+/* One Model part (Model.h:27,30,34): triangles in model space + its border box. */
+struct rtg_triangle;
+typedef struct rtg_part {
+    const struct rtg_triangle *tris; /* the part's triangles (Model::parts[p]) */
+    int32_t tri_count; /* <= 32767: clTri::numb is int16 (3DElement.h:125) */
+    int32_t mtl;       /* index into materials (part_mtl -> mtls, flattened) */
+    rtg_vec4 bmin, bmax; /* Model::borders[2*part], [2*part+1] (model space) */
+} rtg_part;
+
+/* Triangle (3DElement.h:128-139): points and normals in model space.  Same
+ * 128-byte layout as the reference's Triangle. */
+typedef struct rtg_triangle {
+    rtg_vec4 p[3];
+    rtg_vec4 n[3];
+    float uv[6];
+    float pad[2];
+} rtg_triangle;
+
+/* Light (3DElement.h:204-225). */
+typedef struct rtg_light {
+    int32_t type; /* RTG_LIGHT_* */
+    int32_t on;   /* Light::bLight */
+    int32_t pad[2];
+    rtg_vec4 position, ambient, diffuse, specular, attenuation;
+} rtg_light;
+
+/* Camera (3DElement.h:227-237). */
+typedef struct rtg_camera {
+    rtg_vec4 position, u, v, n;
+    int32_t width, height;
+    float fovy, zNear, zFar;
+    float pad[3];
+} rtg_camera;
+
+/* A whole Scene (Scene.h:16-48), flattened in Scene::Objects order. */
+typedef struct rtg_scene_desc {
+    const rtg_object *objects;
+    int32_t n_objects;
+    int32_t n_parts;
+    const rtg_part *parts;
+    int64_t n_tris; /* sum of parts[i].tri_count */
+    const rtg_material *materials;
+    int32_t n_materials;
+    int32_t n_textures;
+    const rtg_texture *textures;
+    const rtg_light *lights;
+    int32_t n_lights;
+    int32_t pad;
+    rtg_vec4 env_light; /* Scene::EnvLight */
+    rtg_camera camera;  /* Scene::cam */
+} rtg_scene_desc;
+
+/* Per-frame render parameters: RayTracer::start(type, tnum) (RayTracer.cpp:614)
+ * plus what the reference fixes implicitly. */
+typedef struct rtg_params {
+    int32_t mode;        /* RTG_MODE_* (only RTG_MODE_RAYTRACE / REFRACTTEST render the full tree) */
+    int32_t max_level;   /* RayTracer::maxLevel: levels 0..max_level are traced */
+    int32_t width, height; /* 0: take the scene camera's */
+    int32_t spp;         /* 1: the reference's single pixel-corner sample; 4: 2x2 jittered strata */
+    int32_t tile_compat; /* 1: write only whole 64x64 tiles, others stay 127 (RayTracer.cpp:13-21,620) */
+    int32_t engine;      /* RTG_ENGINE_* */
+    int32_t flags;       /* RTG_FLAG_* */
+    uint64_t seed;       /* jitter seed for spp > 1 */
+} rtg_params;
+
+/* Ray counts use the reference recursion's semantics (RayTracer.cpp:450-596). */
+typedef struct rtg_stats {
+    uint64_t rays_primary, rays_shadow, rays_reflect, rays_refract;
+    uint64_t rays_total;
+    double render_ms; /* device time of the render, HIP events */
+    double build_ms;  /* device time of the last BVH build for this scene */
+    uint64_t retests_fp64;
+} rtg_stats;
+
+typedef struct rtg_scene_s *rtg_scene_t;
+typedef struct rtg_job_s *rtg_job_t;
+
+/* ---- library ---------------------------------------------------------- */
+
+/* Replaces RayTracer::RayTracer (RayTracer.cpp:600-606): select the device
+ * and create the library's stream.  Idempotent per device. */
+int rtg_init(int device);
+/* Release every library resource (scenes must be destroyed first). */
+int rtg_shutdown(void);
+/* Thread-local message of the last failing call (never NULL). */
+const char *rtg_last_error(void);
+int rtg_abi_version(void);
+
+/* ---- scene residency -------------------------------------------------- */
+
+/* Replaces RayTracer::start's per-frame `RTPrepare` of every shown object
+ * (RayTracer.cpp:621-625, Model::RTPrepare Model.cpp:402-480): upload the
+ * flattened scene to HBM and build the BVH over spheres and Model triangles on
+ * the device.  The scene stays resident until rtg_scene_destroy. */
+int rtg_scene_create(const rtg_scene_desc *desc, rtg_scene_t *out);
+/* Replaces the effect of Scene mutations (Scene.cpp:157-301) on the render:
+ * re-upload `desc` into an existing handle (rebuilds the BVH). */
+int rtg_scene_update(rtg_scene_t scene, const rtg_scene_desc *desc);
+int rtg_scene_destroy(rtg_scene_t scene);
+/* Device bytes held by the scene (geometry + BVH). */
+int rtg_scene_bytes(rtg_scene_t scene, uint64_t *bytes);
+/* Device time (ms) of the scene's last BVH build. */
+int rtg_scene_build_ms(rtg_scene_t scene, double *ms);
+
+/* ---- render ----------------------------------------------------------- */
+
+/* Replaces RayTracer::start + parallelRT + RTfrac (RayTracer.cpp:5-45,
+ * 450-596, 614-696) for one frame, blocking.  `rgb` receives W*H*3 bytes.
+ * `ids` (optional, W*H*2 int32) receives each pixel's primary hit as
+ * (object index, triangle index within the model or -1), (-1,-1) for a miss.
+ * `tprim` (optional, W*H floats) receives the primary hit distance (1e20 miss). */
+int rtg_render(rtg_scene_t scene, const rtg_params *params, uint8_t *rgb,
+               int32_t *ids, float *tprim, rtg_stats *stats);
+
+/* Same, with device pointers for the outputs and a caller stream (hipStream_t
+ * passed as void*, NULL: the library stream).  Asynchronous with respect to
+ * the host; `stats` ray counts are filled when the stream completes only if
+ * `stats_dev` is a device pointer to 4 uint64 (primary, shadow, reflect,
+ * refract), otherwise not at all. */
+int rtg_render_device(rtg_scene_t scene, const rtg_params *params, uint8_t *rgb_dev,
+                      int32_t *ids_dev, float *tprim_dev, uint64_t *stats_dev, void *stream);
+
+/* Non-blocking render: RayTracer::start's detached workers + monitor
+ * (RayTracer.cpp:664-695).  rtg_poll returns RTG_E_PENDING while running, RTG_OK
+ * when finished (isFinish, RayTracer.h:47), RTG_E_CANCELLED after rtg_cancel.
+ * Output buffers must stay valid until rtg_job_wait returns. */
+int rtg_render_async(rtg_scene_t scene, const rtg_params *params, uint8_t *rgb,
+                     int32_t *ids, float *tprim, rtg_job_t *job);
+int rtg_poll(rtg_job_t job);
+/* Replaces RayTracer::stop (RayTracer.cpp:698-701): cooperative cancel. Tiles
+ * not yet rendered keep the value 127, as in the reference. */
+int rtg_cancel(rtg_job_t job);
+/* Wait for completion and fill `stats` (may be NULL); returns the job's status. */
+int rtg_job_wait(rtg_job_t job, rtg_stats *stats);
+/* Release a job (waits for it first if it is still running). */
+int rtg_job_free(rtg_job_t job);
+
+/* ---- batched rendering (many requests, one persistent launch) --------- */
+
+typedef struct rtg_request {
+    rtg_scene_t scene;
+    rtg_params params;
+    uint8_t *rgb;   /* host W*H*3 */
+    int32_t *ids;   /* optional host W*H*2 */
+} rtg_request;
+
+/* Render n requests of uneven size and scene together: all their rays share
+ * one global queue per depth, processed by persistent kernels, replayed from
+ * a captured hipGraph when the batch shape repeats.  Each request's result is
+ * identical to rendering it alone with rtg_render. */
+int rtg_render_batch(const rtg_request *reqs, int n, rtg_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RTG_H */

@@ -1,0 +1,177 @@
+"""raytrace_amd — MI355X-native render path of the RayTrace Whitted renderer.
+
+Python face of librtg (include/rtg.h, include/rtg_host.h):
+
+    import raytrace_amd as rt
+    rt.init(0)
+    hs = rt.HostScene.workload(1)             # seeded scene, built through the Scene API
+    ds = rt.DeviceScene(hs)                   # resident in HBM, BVH built on the device
+    out = ds.render(rt.params(640, 480, max_level=3))
+    out["rgb"]  # (H, W, 3) uint8, row 0 = bottom image row as in the reference
+
+There is no CPU fallback: every render runs the HIP kernels of librtg.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _capi
+from ._capi import (RTG_ENGINE_AUTO, RTG_ENGINE_MEGAKERNEL, RTG_ENGINE_WAVEFRONT, RTG_FLAG_FP64_RETEST,
+                    RTG_MODE_RAYTRACE, RtgError, check, rtg_gen_info, rtg_gen_opts, rtg_params, rtg_stats)
+
+__all__ = [
+    "init", "shutdown", "params", "HostScene", "DeviceScene", "RtgError", "RTG_ENGINE_AUTO",
+    "RTG_ENGINE_MEGAKERNEL", "RTG_ENGINE_WAVEFRONT", "RTG_FLAG_FP64_RETEST", "RTG_MODE_RAYTRACE",
+]
+
+
+def init(device: int = 0) -> None:
+    check(_capi.load().rtg_init(device), "rtg_init")
+
+
+def shutdown() -> None:
+    check(_capi.load().rtg_shutdown(), "rtg_shutdown")
+
+
+def params(width: int, height: int, max_level: int = 1, spp: int = 1, tile_compat: int = 0,
+           engine: int = RTG_ENGINE_AUTO, flags: int = 0, seed: int = 0, mode: int = RTG_MODE_RAYTRACE) -> rtg_params:
+    return rtg_params(mode=mode, max_level=max_level, width=width, height=height, spp=spp, tile_compat=tile_compat,
+                      engine=engine, flags=flags, seed=seed)
+
+
+class HostScene:
+    """A reference-API Scene (raytrace_amd/host) owned through the C ABI."""
+
+    def __init__(self, handle: C.c_void_p, info: rtg_gen_info | None = None):
+        self._lib = _capi.load()
+        self.h = handle
+        self.info = info
+
+    @classmethod
+    def new(cls) -> "HostScene":
+        lib = _capi.load()
+        h = C.c_void_p()
+        check(lib.rtg_host_scene_new(C.byref(h)), "rtg_host_scene_new")
+        return cls(h)
+
+    @classmethod
+    def workload(cls, config: int, seed: int = 0, width: int = 0, height: int = 0, detail: int = 0,
+                 n_spheres: int = -1, n_lights: int = -1, index: int = 0, max_level: int = -1) -> "HostScene":
+        lib = _capi.load()
+        o = rtg_gen_opts(seed=seed, width=width, height=height, detail=detail, n_spheres=n_spheres,
+                         n_lights=n_lights, index=index, max_level=max_level)
+        h = C.c_void_p()
+        info = rtg_gen_info()
+        check(lib.rtg_host_scene_config(config, C.byref(o), C.byref(h), C.byref(info)), "rtg_host_scene_config")
+        return cls(h, info)
+
+    @property
+    def desc_ptr(self) -> int:
+        p = self._lib.rtg_host_scene_desc(self.h)
+        if not p:
+            raise RtgError(-1, "rtg_host_scene_desc")
+        return p
+
+    def close(self) -> None:
+        if self.h:
+            self._lib.rtg_host_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # Scene API (Scene.h:33-45)
+    def add_sphere(self, r: float) -> int:
+        return check(self._lib.rtg_host_add_sphere(self.h, r), "AddSphere")
+
+    def add_plane(self) -> int:
+        return check(self._lib.rtg_host_add_plane(self.h), "AddPlane")
+
+    def add_cube(self, length: float) -> int:
+        return check(self._lib.rtg_host_add_cube(self.h, length), "AddCube")
+
+    def add_ballplane(self, r: float) -> int:
+        return check(self._lib.rtg_host_add_ballplane(self.h, r), "AddBallPlane")
+
+    def add_model_obj(self, obj_path: str, mtl_path: str) -> int:
+        return check(self._lib.rtg_host_add_model_obj(self.h, obj_path.encode(), mtl_path.encode()), "AddModel")
+
+    def add_light(self, type_: int, comp, atte=(1.0, 0.0, 0.0, 1.0)) -> int:
+        return check(self._lib.rtg_host_add_light(self.h, type_, _capi.floats(*comp), _capi.floats(*atte)),
+                     "AddLight")
+
+    def move_pos(self, type_: int, num: int, v) -> None:
+        check(self._lib.rtg_host_move_pos(self.h, type_, num, _capi.floats(*v)), "MovePos")
+
+    def switch(self, type_: int, num: int, show: bool) -> int:
+        return check(self._lib.rtg_host_switch(self.h, type_, num, int(show)), "Switch")
+
+    def set_position(self, num: int, pos) -> None:
+        check(self._lib.rtg_host_set_position(self.h, num, _capi.floats(*pos)), "position")
+
+    def set_light_position(self, num: int, pos) -> None:
+        check(self._lib.rtg_host_set_light_position(self.h, num, _capi.floats(*pos)), "Light::SetProperty")
+
+    def set_env_light(self, env) -> None:
+        check(self._lib.rtg_host_set_env_light(self.h, _capi.floats(*env)), "EnvLight")
+
+    def resize(self, w: int, h: int) -> None:
+        check(self._lib.rtg_host_resize(self.h, w, h), "Camera::resize")
+
+    def raytrace(self, type_: int = RTG_MODE_RAYTRACE, max_level: int = 1):
+        """RayTracer::start + wait for isFinish; returns (2048x2048x3 buffer, useTime)."""
+        out = np.full((2048 * 2048 * 3,), 127, np.uint8)
+        t = C.c_double()
+        check(self._lib.rtg_host_raytrace(self.h, type_, max_level, out.ctypes.data_as(C.c_void_p), C.byref(t)),
+              "RayTracer::start")
+        return out, t.value
+
+
+class DeviceScene:
+    """A scene resident in HBM (rtg_scene_create)."""
+
+    def __init__(self, host: HostScene):
+        self._lib = _capi.load()
+        self.h = C.c_void_p()
+        check(self._lib.rtg_scene_create(C.c_void_p(host.desc_ptr), C.byref(self.h)), "rtg_scene_create")
+
+    def update(self, host: HostScene) -> None:
+        check(self._lib.rtg_scene_update(self.h, C.c_void_p(host.desc_ptr)), "rtg_scene_update")
+
+    @property
+    def build_ms(self) -> float:
+        v = C.c_double()
+        check(self._lib.rtg_scene_build_ms(self.h, C.byref(v)), "rtg_scene_build_ms")
+        return v.value
+
+    @property
+    def nbytes(self) -> int:
+        v = C.c_uint64()
+        check(self._lib.rtg_scene_bytes(self.h, C.byref(v)), "rtg_scene_bytes")
+        return v.value
+
+    def render(self, p: rtg_params, ids: bool = True, tprim: bool = True) -> dict:
+        W, H = p.width, p.height
+        rgb = np.empty((H, W, 3), np.uint8)
+        idb = np.empty((H, W, 2), np.int32) if ids else None
+        tb = np.empty((H, W), np.float32) if tprim else None
+        st = rtg_stats()
+        ptr = lambda a: a.ctypes.data_as(C.c_void_p) if a is not None else None
+        check(self._lib.rtg_render(self.h, C.byref(p), ptr(rgb), ptr(idb), ptr(tb), C.byref(st)), "rtg_render")
+        return {"rgb": rgb, "ids": idb, "t": tb, "stats": st}
+
+    def close(self) -> None:
+        if self.h:
+            self._lib.rtg_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

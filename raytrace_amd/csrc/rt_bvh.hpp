@@ -1,0 +1,188 @@
+// rt_bvh.hpp — per-element steps of the device LBVH build.
+//
+// Build (one kernel per step, rtg_bvh.hip):
+//   1. prim_setup      triangle -> clTri values (p0 + position, p1 - p0,
+//                      p2 - p0, Model.cpp:426), octant-bin membership
+//                      (Model.cpp:430-465), shading arrays, and an AABB of
+//                      the implied vertices computed in FP64 and rounded
+//                      outward to FP32, then padded by 2^-20 of its magnitude
+//                      (covers the FP32 Moller-Trumbore acceptance fuzz).
+//   2. morton          30-bit Morton code of the box centre; key = code<<32 | id
+//                      (unique keys).
+//   3. radix sort of the keys.
+//   4. karras_node     binary radix tree (Karras 2012) over the sorted keys.
+//   5. refit           bottom-up box union (exact: min/max).
+//   6. emit            subtrees of <= kLeafMax primitives become leaves; the
+//                      rest are written as 64-byte BvhNodes.
+#pragma once
+#include "rt_scene.hpp"
+#include "../../include/rtg.h"
+
+namespace rtg {
+
+constexpr int kLeafMax = 4;
+
+struct Box6 {
+    float lo[3], hi[3];
+};
+
+RTG_HD Box6 box_union(const Box6 &a, const Box6 &b) {
+    Box6 r;
+    for (int k = 0; k < 3; ++k) {
+        r.lo[k] = fminf(a.lo[k], b.lo[k]);
+        r.hi[k] = fmaxf(a.hi[k], b.hi[k]);
+    }
+    return r;
+}
+
+RTG_HD float round_down(double x) {
+    float f = (float)x;
+    return (double)f > x ? nextafterf(f, -INFINITY) : f;
+}
+RTG_HD float round_up(double x) {
+    float f = (float)x;
+    return (double)f < x ? nextafterf(f, INFINITY) : f;
+}
+
+// FP64 box of points, rounded outward to FP32 and padded.
+RTG_HD Box6 padded_box(const double lo[3], const double hi[3]) {
+    Box6 b;
+    float m = 1.0f;
+    for (int k = 0; k < 3; ++k) {
+        b.lo[k] = round_down(lo[k]);
+        b.hi[k] = round_up(hi[k]);
+        m = fmaxf(m, fmaxf(fabsf(b.lo[k]), fabsf(b.hi[k])));
+    }
+    const float pad = m * 9.5367431640625e-07f;  // 2^-20
+    for (int k = 0; k < 3; ++k) {
+        b.lo[k] = b.lo[k] - pad;
+        b.hi[k] = b.hi[k] + pad;
+    }
+    return b;
+}
+
+// Per-triangle setup.  `tri` is the model-space reference Triangle,
+// `pos` the model position, `mid` the part's model-space border midpoint
+// ((borders[2p] + borders[2p+1]) * 0.5, Model.cpp:420).
+RTG_HD void prim_setup_tri(const rtg_triangle &tri, V3 pos, V3 mid, int g, int obj, float4 *prim3, float4 *nrm3,
+                           float2 *uv3, uint8_t &bins, Box6 &box) {
+    const V3 p0 = V3{tri.p[0].x, tri.p[0].y, tri.p[0].z}, p1 = V3{tri.p[1].x, tri.p[1].y, tri.p[1].z},
+             p2 = V3{tri.p[2].x, tri.p[2].y, tri.p[2].z};
+    const V3 e1 = p1 - p0, e2 = p2 - p0, pw = p0 + pos;
+    prim3[0] = make_float4(pw.x, pw.y, pw.z, as_float(g));
+    prim3[1] = make_float4(e1.x, e1.y, e1.z, as_float(obj));
+    prim3[2] = make_float4(e2.x, e2.y, e2.z, 0.0f);
+    for (int k = 0; k < 3; ++k) {
+        nrm3[k] = make_float4(tri.n[k].x, tri.n[k].y, tri.n[k].z, 0.0f);
+        uv3[k] = make_float2(tri.uv[2 * k], tri.uv[2 * k + 1]);
+    }
+    // octant bins by the model-space triangle bounds (_mm_min_ps/_mm_max_ps)
+    float mn[3], mx[3];
+    for (int k = 0; k < 3; ++k) {
+        const float a0 = (&tri.p[0].x)[k], a1 = (&tri.p[1].x)[k], a2 = (&tri.p[2].x)[k];
+        const float m12 = sse_min(a1, a2), M12 = sse_max(a1, a2);
+        mn[k] = sse_min(a0, m12);
+        mx[k] = sse_max(a0, M12);
+    }
+    const float md[3] = {mid.x, mid.y, mid.z};
+    uint32_t in = 0;
+    if (mn[0] <= md[0]) {
+        if (mn[2] <= md[2]) in |= (mn[1] <= md[1] ? 1u : 0u) | (mx[1] >= md[1] ? 2u : 0u);
+        if (mx[2] >= md[2]) in |= (mn[1] <= md[1] ? 4u : 0u) | (mx[1] >= md[1] ? 8u : 0u);
+    }
+    if (mx[0] >= md[0]) {
+        if (mn[2] <= md[2]) in |= (mn[1] <= md[1] ? 16u : 0u) | (mx[1] >= md[1] ? 32u : 0u);
+        if (mx[2] >= md[2]) in |= (mn[1] <= md[1] ? 64u : 0u) | (mx[1] >= md[1] ? 128u : 0u);
+    }
+    bins = (uint8_t)in;
+    // implied world vertices p0w, p0w + e1, p0w + e2, exact in FP64
+    double lo[3], hi[3];
+    const double w0[3] = {pw.x, pw.y, pw.z}, d1[3] = {e1.x, e1.y, e1.z}, d2[3] = {e2.x, e2.y, e2.z};
+    for (int k = 0; k < 3; ++k) {
+        const double a = w0[k], b = w0[k] + d1[k], c = w0[k] + d2[k];
+        lo[k] = fmin(a, fmin(b, c));
+        hi[k] = fmax(a, fmax(b, c));
+    }
+    box = padded_box(lo, hi);
+}
+
+RTG_HD void prim_setup_sphere(V3 c, float radius, float rsq, int obj, float4 *prim3, Box6 &box) {
+    prim3[0] = make_float4(c.x, c.y, c.z, as_float(~obj));
+    prim3[1] = make_float4(rsq, radius, 0.0f, 0.0f);
+    prim3[2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const double r = fabs((double)radius);
+    const double lo[3] = {(double)c.x - r, (double)c.y - r, (double)c.z - r};
+    const double hi[3] = {(double)c.x + r, (double)c.y + r, (double)c.z + r};
+    box = padded_box(lo, hi);
+}
+
+RTG_HD uint32_t expand_bits10(uint32_t v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+RTG_HD uint64_t morton_key(const Box6 &b, const float cmin[3], const float cinv[3], int id) {
+    uint32_t q[3];
+    for (int k = 0; k < 3; ++k) {
+        float c = 0.5f * (b.lo[k] + b.hi[k]);
+        float x = (c - cmin[k]) * cinv[k] * 1024.0f;
+        x = fminf(fmaxf(x, 0.0f), 1023.0f);
+        q[k] = (uint32_t)x;
+    }
+    const uint64_t code = (expand_bits10(q[0]) << 2) | (expand_bits10(q[1]) << 1) | expand_bits10(q[2]);
+    return (code << 32) | (uint32_t)id;
+}
+
+RTG_HD int clz64(uint64_t x) { return x == 0 ? 64 : __builtin_clzll(x); }
+
+RTG_HD int delta(const uint64_t *keys, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    return clz64(keys[i] ^ keys[j]);
+}
+
+// Karras 2012, internal node i of n-1.  Children: >= 0 internal node index,
+// < 0 leaf ~k (sorted position k).  Writes the node's key range [first, last].
+RTG_HD void karras_node(const uint64_t *keys, int n, int i, int &left, int &right, int &first, int &last) {
+    const int d = delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1) >= 0 ? 1 : -1;
+    const int dmin = delta(keys, n, i, i - d);
+    int lmax = 2;
+    while (delta(keys, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for (int t = lmax / 2; t >= 1; t /= 2)
+        if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    const int j = i + l * d;
+    const int dnode = delta(keys, n, i, j);
+    int s = 0;
+    for (int div = 2;; div *= 2) {
+        const int t = (l + div - 1) / div;
+        if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+        if (t <= 1) break;
+    }
+    const int gamma = i + s * d + (d < 0 ? -1 : 0);
+    first = i < j ? i : j;
+    last = i < j ? j : i;
+    left = first == gamma ? ~gamma : gamma;
+    right = last == gamma + 1 ? ~(gamma + 1) : gamma + 1;
+}
+
+// Child link and box of a radix-tree child for the emitted layout.
+RTG_HD int emit_link(int child, const int *first, const int *last, const int *newidx) {
+    if (child < 0) return make_leaf(~child, 1);
+    const int cnt = last[child] - first[child] + 1;
+    if (cnt <= kLeafMax) return make_leaf(first[child], cnt);
+    return newidx[child];
+}
+
+RTG_HD BvhNode make_node(const Box6 &b0, const Box6 &b1, int l0, int l1) {
+    BvhNode n;
+    n.xy0 = make_float4(b0.lo[0], b0.hi[0], b0.lo[1], b0.hi[1]);
+    n.xy1 = make_float4(b1.lo[0], b1.hi[0], b1.lo[1], b1.hi[1]);
+    n.z01 = make_float4(b0.lo[2], b0.hi[2], b1.lo[2], b1.hi[2]);
+    n.link = make_int4(l0, l1, 0, 0);
+    return n;
+}
+
+}  // namespace rtg

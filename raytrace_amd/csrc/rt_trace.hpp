@@ -1,0 +1,281 @@
+// rt_trace.hpp — closest-hit and any-hit queries over a SceneView.
+//
+// Semantics are those of the reference's per-object scan (RayTracer.cpp:
+// 458-465 and 510-520): the closest hit is the smallest t, ties going to the
+// lower object index (an object replaces the running hit only when strictly
+// closer), and each primitive keeps its own acceptance rule and self-skip
+// rule.  The BVH only culls; its boxes are conservative (FP64-expanded leaf
+// bounds, robust slab test), so the result equals the scan's.
+//
+// Self-skip of triangles.  The reference tests the clTri *copies* of a
+// model's octant bins and skips only the copy whose address is the ray's
+// source (Model.cpp:775); a triangle stored in several bins can therefore be
+// hit again by its own secondary rays.  `self_testable` reproduces that: the
+// source triangle is tested iff the ray's BorderTest / BorderTestEx reach a
+// bin holding another copy of it.
+#pragma once
+#include "rt_scene.hpp"
+
+namespace rtg {
+
+enum : int { RAY_BASE = 1, RAY_SHADOW = 2, RAY_REFLECT = 3, RAY_REFRACT = 4 };
+enum : int { OBJ_SPHERE = 1, OBJ_CUBE = 2, OBJ_MODEL = 3, OBJ_PLANE = 4, OBJ_BALLPLANE = 5 };
+enum : int { HIT_NONE = 0, HIT_TRI = 1, HIT_SPHERE = 2, HIT_PLANE = 3, HIT_CUBE = 4, HIT_BALL = 5 };
+
+struct TraceRay {
+    Ray r;
+    int type;
+    int inside;     // Ray::isInside
+    SrcId src;      // HitRes::obj of the parent hit
+    int self_test;  // source triangle may be hit again (see header)
+};
+
+struct Hit {
+    float t;
+    int kind;
+    int obj, prim;
+    float w, u, v;  // barycentrics (triangles)
+};
+
+RTG_HD uint64_t hit_key(int obj, int prim) { return ((uint64_t)(uint32_t)obj << 32) | (uint32_t)(prim + 1); }
+
+// 1 + 2*gamma(3) with margin (Ize 2013): scaling the far slab distance makes
+// the FP32 slab test conservative.
+constexpr float kFarScale = 1.0000006f;
+
+RTG_HD V3 safe_inv(V3 d) {
+    return V3{d.x != 0.0f ? 1.0f / d.x : copysignf(1e30f, d.x), d.y != 0.0f ? 1.0f / d.y : copysignf(1e30f, d.y),
+              d.z != 0.0f ? 1.0f / d.z : copysignf(1e30f, d.z)};
+}
+
+RTG_HD bool slab(V3 o, V3 inv, float lx, float hx, float ly, float hy, float lz, float hz, float tmax,
+                 float &tnear) {
+    float ax = (lx - o.x) * inv.x, bx = (hx - o.x) * inv.x;
+    float ay = (ly - o.y) * inv.y, by = (hy - o.y) * inv.y;
+    float az = (lz - o.z) * inv.z, bz = (hz - o.z) * inv.z;
+    float n = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), 0.0f));
+    float f = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz)) * kFarScale;
+    tnear = n;
+    return n <= fminf(f, tmax);
+}
+
+// Host emulation stack (the device uses an LDS-backed stack, rtg_kernels.hip).
+struct ArrayStack {
+    int d[128];
+    int sp = 0;
+    RTG_HD void push(int v) { d[sp++] = v; }
+    RTG_HD int pop() { return d[--sp]; }
+    RTG_HD bool empty() const { return sp == 0; }
+};
+
+// Visit the primitives of every leaf the segment [0, tmax] reaches, nearer
+// child first.  visit(prim_index, tmax&) returns true to stop.
+template <class Stack, class Visit>
+RTG_HD void bvh_walk(const SceneView &S, const Ray &r, float &tmax, Stack &st, Visit &&visit) {
+    if (S.n_prims == 0) return;
+    const V3 inv = safe_inv(r.d);
+    int node = S.root;
+    for (;;) {
+        while (!is_leaf(node)) {
+            const BvhNode &n = S.nodes[node];
+            const float4 a = n.xy0, b = n.xy1, z = n.z01;
+            const int4 l = n.link;
+            float t0, t1;
+            const bool h0 = slab(r.o, inv, a.x, a.y, a.z, a.w, z.x, z.y, tmax, t0);
+            const bool h1 = slab(r.o, inv, b.x, b.y, b.z, b.w, z.z, z.w, tmax, t1);
+            if (h0 && h1) {
+                const bool first0 = t0 <= t1;
+                st.push(first0 ? l.y : l.x);
+                node = first0 ? l.x : l.y;
+            } else if (h0) {
+                node = l.x;
+            } else if (h1) {
+                node = l.y;
+            } else {
+                if (st.empty()) return;
+                node = st.pop();
+            }
+        }
+        const int first = leaf_first(node), count = leaf_count(node);
+        for (int i = 0; i < count; ++i)
+            if (visit(first + i, tmax)) return;
+        if (st.empty()) return;
+        node = st.pop();
+    }
+}
+
+// Copy bin the reference records for a hit of ray `r` (source `src`) on
+// triangle g: the first bin in scan order that the ray's BorderTestEx mask
+// reaches and that holds a copy not skipped by the ray (Model.cpp:765-789).
+RTG_HD int record_bin(const SceneView &S, const Ray &r, const SrcId &src, int g) {
+    const DevPart &P = S.parts[S.tri_part[g]];
+    uint32_t K;
+    border_test_ex(r, P.bbmin, P.bbmax, K);
+    const uint32_t member = S.tri_bins[g];
+    uint32_t m = member & K;
+    if (src.prim == g && src.obj == P.obj) m &= ~(1u << src.bin);
+    if (m == 0) m = member;
+    return m ? __builtin_ctz(m) : 0;
+}
+
+// Whether a secondary ray may hit its own source triangle (see header).
+RTG_HD int self_testable(const SceneView &S, const Ray &r, const SrcId &src, float dist_in) {
+    if (src.obj < 0 || src.prim < 0 || S.objs[src.obj].type != OBJ_MODEL) return 0;
+    const DevObject &O = S.objs[src.obj];
+    if (!(border_test(r, O.bmin, O.bmax) < dist_in)) return 0;
+    const DevPart &P = S.parts[S.tri_part[src.prim]];
+    uint32_t K;
+    if (!(border_test_ex(r, P.bbmin, P.bbmax, K) < dist_in)) return 0;
+    return (S.tri_bins[src.prim] & K & ~(1u << src.bin)) != 0;
+}
+
+// Centre of BallPlane sphere `k` (0..15), by the reference's float loop
+// (Basic3DObject.cpp:492-497).
+RTG_HD bool ballplane_center(const DevObject &o, int k, V3 &c) {
+    int cnt = 0;
+    for (float cx = o.radius * -6; cx < o.radius * 8; cx += o.radius * 4)
+        for (float cy = o.radius * -6; cy < o.radius * 8; cy += o.radius * 4) {
+            if (cnt == k) {
+                c = (o.axisx * cx + o.axisy * cy) + o.pos;
+                return true;
+            }
+            ++cnt;
+        }
+    return false;
+}
+
+// Objects outside the BVH, in object order.  Returns true for an any-hit
+// query when blocked.
+RTG_HD bool brute_objects(const SceneView &S, const TraceRay &q, bool anyhit, float dis, Hit &best) {
+    for (int k = 0; k < S.n_brute; ++k) {
+        const int oi = S.brute[k];
+        const DevObject &o = S.objs[oi];
+        const float lim = anyhit ? dis : best.t;
+        if (o.type == OBJ_PLANE) {
+            if (q.src.obj == oi) continue;
+            float t = plane_t(q.r, o.pos, o.normal);
+            if (t < lim) {
+                if (anyhit) return true;
+                best = Hit{t, HIT_PLANE, oi, -1, 0, 0, 0};
+            }
+        } else if (o.type == OBJ_CUBE) {
+            if (q.src.obj == oi) continue;
+            float t = border_test(q.r, o.bmin + o.pos, o.bmax + o.pos);
+            if (t < lim && t > 1e-6f) {
+                if (anyhit) return true;
+                best = Hit{t, HIT_CUBE, oi, -1, 0, 0, 0};
+            }
+        } else if (o.type == OBJ_BALLPLANE) {
+            int cnt = 0;
+            float run = lim;
+            for (float cx = o.radius * -6; cx < o.radius * 8; cx += o.radius * 4)
+                for (float cy = o.radius * -6; cy < o.radius * 8; cy += o.radius * 4) {
+                    const int k2 = cnt++;
+                    const V3 c = (o.axisx * cx + o.axisy * cy) + o.pos;
+                    float t;
+                    if (q.src.obj == oi && q.src.prim == k2) {
+                        if (!q.inside) continue;
+                        if (anyhit) {
+                            if (o.radius < dis) return true;
+                            continue;
+                        }
+                        t = sphere_exit_t(q.r, c, o.rsq);
+                    } else {
+                        t = sphere_enter_t(q.r, c, o.rsq);
+                    }
+                    if (t < run && t > 1e-6f) {
+                        if (anyhit) return true;
+                        run = t;
+                        best = Hit{t, HIT_BALL, oi, k2, 0, 0, 0};
+                    }
+                }
+        }
+    }
+    return false;
+}
+
+// Closest hit (RayTracer.cpp:455-465).
+template <class Stack>
+RTG_HD Hit trace_closest(const SceneView &S, const TraceRay &q, Stack &st) {
+    Hit best{kMiss, HIT_NONE, -1, -1, 0, 0, 0};
+    brute_objects(S, q, false, 0.0f, best);
+    uint64_t bkey = best.kind != HIT_NONE ? hit_key(best.obj, best.prim) : ~0ull;
+    float tmax = best.t;
+    bvh_walk(S, q.r, tmax, st, [&](int pi, float &tm) -> bool {
+        const float4 A = S.prims[3 * pi];
+        const int tag = as_int(A.w);
+        const V3 p = V3{A.x, A.y, A.z};
+        if (tag >= 0) {  // triangle, tag = global index
+            const float4 B = S.prims[3 * pi + 1], C = S.prims[3 * pi + 2];
+            const int obj = as_int(B.w);
+            if (tag == q.src.prim && obj == q.src.obj && !q.self_test) return false;
+            float w, u, v;
+            const float t = tri_test(q.r, V3{B.x, B.y, B.z}, V3{C.x, C.y, C.z}, p, w, u, v);
+            if (t < kMiss) {
+                const uint64_t k = hit_key(obj, tag);
+                if (t < best.t || (t == best.t && k < bkey)) {
+                    best = Hit{t, HIT_TRI, obj, tag, w, u, v};
+                    bkey = k;
+                    tm = t;
+                }
+            }
+        } else {  // sphere, tag = ~object
+            const int obj = ~tag;
+            const float4 B = S.prims[3 * pi + 1];
+            float t;
+            if (q.src.obj == obj) {
+                if (!q.inside) return false;
+                t = sphere_exit_t(q.r, p, B.x);
+            } else {
+                t = sphere_enter_t(q.r, p, B.x);
+            }
+            if (t > 1e-6f && t < kMiss) {
+                const uint64_t k = hit_key(obj, -1);
+                if (t < best.t || (t == best.t && k < bkey)) {
+                    best = Hit{t, HIT_SPHERE, obj, -1, 0, 0, 0};
+                    bkey = k;
+                    tm = t;
+                }
+            }
+        }
+        return false;
+    });
+    return best;
+}
+
+// Any hit closer than `dis` (shadow test, RayTracer.cpp:510-520).
+template <class Stack>
+RTG_HD bool trace_any(const SceneView &S, const TraceRay &q, float dis, Stack &st) {
+    Hit dummy{kMiss, HIT_NONE, -1, -1, 0, 0, 0};
+    if (brute_objects(S, q, true, dis, dummy)) return true;
+    bool blocked = false;
+    float tmax = dis;
+    bvh_walk(S, q.r, tmax, st, [&](int pi, float &) -> bool {
+        const float4 A = S.prims[3 * pi];
+        const int tag = as_int(A.w);
+        const V3 p = V3{A.x, A.y, A.z};
+        float t;
+        if (tag >= 0) {
+            const float4 B = S.prims[3 * pi + 1], C = S.prims[3 * pi + 2];
+            const int obj = as_int(B.w);
+            if (tag == q.src.prim && obj == q.src.obj && !q.self_test) return false;
+            float w, u, v;
+            t = tri_test(q.r, V3{B.x, B.y, B.z}, V3{C.x, C.y, C.z}, p, w, u, v);
+        } else {
+            const int obj = ~tag;
+            const float4 B = S.prims[3 * pi + 1];
+            if (q.src.obj == obj) {
+                if (!q.inside) return false;
+                blocked = B.y < dis;  // HitRes(radius) (Basic3DObject.cpp:143-144)
+                return blocked;
+            }
+            t = sphere_enter_t(q.r, p, B.x);
+            if (!(t > 1e-6f)) return false;
+        }
+        blocked = t < dis;
+        return blocked;
+    });
+    return blocked;
+}
+
+}  // namespace rtg

@@ -1,0 +1,390 @@
+// rtg_api.hip — implementation of the librtg C ABI (include/rtg.h).
+#include <chrono>
+#include <cstring>
+#include <memory>
+
+#include "rtg_internal.hpp"
+
+namespace rtg {
+
+namespace {
+thread_local std::string g_err = "";
+std::mutex g_init_lock;
+std::unique_ptr<Arena> g_arena;
+
+__global__ void k_fill_u8(uint8_t *p, size_t n, uint8_t v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_zero_u64(unsigned long long *p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+uint64_t hash_bytes(uint64_t h, const void *p, size_t n) {
+    const uint8_t *c = static_cast<const uint8_t *>(p);
+    size_t i = 0;
+    uint64_t lanes[4] = {h ^ 0x9E3779B97F4A7C15ull, h + 0xC2B2AE3D27D4EB4Full, h * 3 + 1, ~h};
+    for (; i + 32 <= n; i += 32)
+        for (int k = 0; k < 4; ++k) {
+            uint64_t w;
+            memcpy(&w, c + i + 8 * k, 8);
+            lanes[k] = (lanes[k] ^ w) * 0x100000001B3ull;
+            lanes[k] ^= lanes[k] >> 29;
+        }
+    uint64_t t = n;
+    for (; i < n; ++i) t = t * 131 + c[i];
+    return mix64(lanes[0] ^ mix64(lanes[1] ^ mix64(lanes[2] ^ mix64(lanes[3] ^ t))));
+}
+
+// Hash of everything the BVH depends on.
+uint64_t geometry_hash(const rtg_scene_desc &d) {
+    uint64_t h = hash_bytes(0, d.objects, sizeof(rtg_object) * (size_t)d.n_objects);
+    for (int i = 0; i < d.n_parts; ++i) {
+        const rtg_part &p = d.parts[i];
+        h = hash_bytes(h, &p.tri_count, sizeof(int32_t) * 2);
+        h = hash_bytes(h, &p.bmin, sizeof(rtg_vec4) * 2);
+        if (p.tris && p.tri_count > 0) h = hash_bytes(h, p.tris, sizeof(rtg_triangle) * (size_t)p.tri_count);
+    }
+    return h;
+}
+
+int validate_desc(const rtg_scene_desc *d) {
+    if (d == nullptr) return fail(RTG_E_INVALID, "scene descriptor is NULL");
+    if (d->n_objects < 0 || d->n_parts < 0 || d->n_materials < 0 || d->n_lights < 0 || d->n_textures < 0)
+        return fail(RTG_E_INVALID, "negative count in scene descriptor");
+    if ((d->n_objects && !d->objects) || (d->n_parts && !d->parts) || (d->n_materials && !d->materials) ||
+        (d->n_lights && !d->lights) || (d->n_textures && !d->textures))
+        return fail(RTG_E_INVALID, "NULL array in scene descriptor");
+    if (d->n_lights > RTG_MAX_LIGHTS) return fail(RTG_E_LIMIT, "more than RTG_MAX_LIGHTS lights");
+    for (int i = 0; i < d->n_materials; ++i)
+        if (d->materials[i].tex >= d->n_textures) return fail(RTG_E_INVALID, "material texture out of range");
+    for (int i = 0; i < d->n_parts; ++i)
+        if (d->parts[i].tri_count > 32767) return fail(RTG_E_LIMIT, "part with more than 32767 triangles");
+    return RTG_OK;
+}
+
+int resolve_size(const SceneRes &S, const rtg_params *p, int &W, int &H) {
+    if (p == nullptr) return fail(RTG_E_INVALID, "params is NULL");
+    W = p->width > 0 ? p->width : S.camera.width;
+    H = p->height > 0 ? p->height : S.camera.height;
+    if (W <= 0 || H <= 0 || W > RTG_MAX_DIM || H > RTG_MAX_DIM) return fail(RTG_E_LIMIT, "frame size out of range");
+    if (p->max_level < 0 || p->max_level > RTG_MAX_LEVEL) return fail(RTG_E_LIMIT, "max_level out of range");
+    if (!(p->spp == 0 || p->spp == 1 || p->spp == 4)) return fail(RTG_E_INVALID, "spp must be 1 or 4");
+    return RTG_OK;
+}
+
+}  // namespace
+
+void set_error(const std::string &msg) { g_err = msg; }
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+int DevBuf::alloc(size_t n) {
+    release();
+    if (n == 0) n = 16;
+    if (hipMalloc(&p, n) != hipSuccess) {
+        p = nullptr;
+        return RTG_E_NOMEM;
+    }
+    bytes = n;
+    return RTG_OK;
+}
+void DevBuf::release() {
+    if (p != nullptr) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+int ensure(DevBuf &b, size_t n) {
+    if (b.p != nullptr && b.bytes >= n) return RTG_OK;
+    return b.alloc(n);
+}
+
+Arena *arena() { return g_arena.get(); }
+
+// Render one frame into device outputs; the caller holds A.lock.
+static int render_device(SceneRes &S, const rtg_params &p, int W, int H, const RenderOut &o, hipStream_t stream,
+                         Arena &A) {
+    const size_t npix = (size_t)W * H;
+    k_fill_u8<<<1024, 256, 0, stream>>>(o.rgb, npix * 3, 127);
+    if (o.ids) k_fill_u8<<<1024, 256, 0, stream>>>(reinterpret_cast<uint8_t *>(o.ids), npix * 8, 0xff);
+    k_zero_u64<<<1, 64, 0, stream>>>(o.counts, 4);
+    if (p.mode != RTG_MODE_RAYTRACE && p.mode != RTG_MODE_REFRACTTEST && p.mode != 0)
+        return fail(RTG_E_INVALID, "render mode not supported by this build");
+    int engine = p.engine == RTG_ENGINE_AUTO ? RTG_ENGINE_MEGAKERNEL : p.engine;
+    if (engine == RTG_ENGINE_WAVEFRONT) return launch_wavefront(S, p, W, H, o, stream, A);
+    return launch_megakernel(S, p, W, H, o, stream, A);
+}
+
+}  // namespace rtg
+
+using namespace rtg;
+
+struct rtg_scene_s {
+    SceneRes res;
+};
+
+struct rtg_job_s {
+    std::thread th;
+    std::atomic<int> status{RTG_E_PENDING};
+    std::atomic<int> cancel{0};
+    std::atomic<int> running{0};
+    rtg_stats stats{};
+};
+
+extern "C" {
+
+const char *rtg_last_error(void) { return g_err.c_str(); }
+int rtg_abi_version(void) { return RTG_ABI_VERSION; }
+
+int rtg_init(int device) {
+    std::lock_guard<std::mutex> g(g_init_lock);
+    if (g_arena && g_arena->device == device) return RTG_OK;
+    if (g_arena) return fail(RTG_E_STATE, "librtg already initialised on another device");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(RTG_E_DEVICE, "no HIP device available");
+    if (device < 0 || device >= n) return fail(RTG_E_INVALID, "device index out of range");
+    RTG_HIP(hipSetDevice(device));
+    std::unique_ptr<Arena> A(new Arena());
+    A->device = device;
+    hipDeviceProp_t prop;
+    RTG_HIP(hipGetDeviceProperties(&prop, device));
+    A->cus = prop.multiProcessorCount;
+    RTG_HIP(hipStreamCreateWithFlags(&A->stream, hipStreamNonBlocking));
+    RTG_HIP(hipEventCreate(&A->ev0));
+    RTG_HIP(hipEventCreate(&A->ev1));
+    RTG_HIP(hipHostMalloc(reinterpret_cast<void **>(&A->cancel_host), sizeof(int), hipHostMallocMapped));
+    *A->cancel_host = 0;
+    RTG_HIP(hipHostGetDevicePointer(reinterpret_cast<void **>(&A->cancel_dev), A->cancel_host, 0));
+    if (ensure(A->counters, 4 * sizeof(unsigned long long))) return fail(RTG_E_NOMEM, "counters");
+    g_arena = std::move(A);
+    return RTG_OK;
+}
+
+int rtg_shutdown(void) {
+    std::lock_guard<std::mutex> g(g_init_lock);
+    if (!g_arena) return RTG_OK;
+    Arena &A = *g_arena;
+    {
+        std::lock_guard<std::mutex> l(A.lock);
+        (void)hipStreamSynchronize(A.stream);
+        A.spill.release();
+        A.tables.release();
+        A.counters.release();
+        A.out_rgb.release();
+        A.out_ids.release();
+        A.out_t.release();
+        (void)hipHostFree(A.cancel_host);
+        (void)hipEventDestroy(A.ev0);
+        (void)hipEventDestroy(A.ev1);
+        (void)hipStreamDestroy(A.stream);
+    }
+    g_arena.reset();
+    return RTG_OK;
+}
+
+int rtg_scene_create(const rtg_scene_desc *desc, rtg_scene_t *out) {
+    if (out == nullptr) return fail(RTG_E_INVALID, "out is NULL");
+    *out = nullptr;
+    int st = validate_desc(desc);
+    if (st != RTG_OK) return st;
+    if (!arena()) return fail(RTG_E_STATE, "call rtg_init first");
+    Arena &A = *arena();
+    std::unique_ptr<rtg_scene_s> s(new rtg_scene_s());
+    s->res.device = A.device;
+    {
+        std::lock_guard<std::mutex> l(A.lock);
+        st = scene_build(s->res, *desc, A.stream);
+    }
+    if (st != RTG_OK) return st;
+    s->res.geom_hash = geometry_hash(*desc);
+    *out = s.release();
+    return RTG_OK;
+}
+
+int rtg_scene_update(rtg_scene_t scene, const rtg_scene_desc *desc) {
+    if (scene == nullptr) return fail(RTG_E_INVALID, "scene is NULL");
+    int st = validate_desc(desc);
+    if (st != RTG_OK) return st;
+    if (!arena()) return fail(RTG_E_STATE, "call rtg_init first");
+    Arena &A = *arena();
+    std::lock_guard<std::mutex> l(A.lock);
+    // Always rebuild: shading tables are tiny and the BVH build is fast; the
+    // geometry hash only records the content for callers that compare it.
+    SceneRes fresh;
+    fresh.device = A.device;
+    st = scene_build(fresh, *desc, A.stream);
+    if (st != RTG_OK) return st;
+    SceneRes &R = scene->res;
+    std::swap(R.nodes.p, fresh.nodes.p), std::swap(R.nodes.bytes, fresh.nodes.bytes);
+    std::swap(R.prims.p, fresh.prims.p), std::swap(R.prims.bytes, fresh.prims.bytes);
+    std::swap(R.objs.p, fresh.objs.p), std::swap(R.objs.bytes, fresh.objs.bytes);
+    std::swap(R.brute.p, fresh.brute.p), std::swap(R.brute.bytes, fresh.brute.bytes);
+    std::swap(R.parts.p, fresh.parts.p), std::swap(R.parts.bytes, fresh.parts.bytes);
+    std::swap(R.tri_nrm.p, fresh.tri_nrm.p), std::swap(R.tri_nrm.bytes, fresh.tri_nrm.bytes);
+    std::swap(R.tri_uv.p, fresh.tri_uv.p), std::swap(R.tri_uv.bytes, fresh.tri_uv.bytes);
+    std::swap(R.tri_part.p, fresh.tri_part.p), std::swap(R.tri_part.bytes, fresh.tri_part.bytes);
+    std::swap(R.tri_bins.p, fresh.tri_bins.p), std::swap(R.tri_bins.bytes, fresh.tri_bins.bytes);
+    std::swap(R.mtls.p, fresh.mtls.p), std::swap(R.mtls.bytes, fresh.mtls.bytes);
+    std::swap(R.lights.p, fresh.lights.p), std::swap(R.lights.bytes, fresh.lights.bytes);
+    std::swap(R.texs.p, fresh.texs.p), std::swap(R.texs.bytes, fresh.texs.bytes);
+    std::swap(R.texels.p, fresh.texels.p), std::swap(R.texels.bytes, fresh.texels.bytes);
+    R.view = fresh.view;
+    R.camera = fresh.camera;
+    R.build_ms = fresh.build_ms;
+    R.bytes = fresh.bytes;
+    R.n_nodes = fresh.n_nodes;
+    R.geom_hash = geometry_hash(*desc);
+    return RTG_OK;
+}
+
+int rtg_scene_destroy(rtg_scene_t scene) {
+    if (scene == nullptr) return RTG_OK;
+    if (arena()) {
+        std::lock_guard<std::mutex> l(arena()->lock);
+        (void)hipStreamSynchronize(arena()->stream);
+    }
+    delete scene;
+    return RTG_OK;
+}
+
+int rtg_scene_bytes(rtg_scene_t scene, uint64_t *bytes) {
+    if (scene == nullptr || bytes == nullptr) return fail(RTG_E_INVALID, "NULL argument");
+    *bytes = scene->res.bytes;
+    return RTG_OK;
+}
+
+int rtg_scene_build_ms(rtg_scene_t scene, double *ms) {
+    if (scene == nullptr || ms == nullptr) return fail(RTG_E_INVALID, "NULL argument");
+    *ms = scene->res.build_ms;
+    return RTG_OK;
+}
+
+static int render_host(rtg_scene_t scene, const rtg_params *params, uint8_t *rgb, int32_t *ids, float *tprim,
+                       rtg_stats *stats, rtg_job_s *job) {
+    if (scene == nullptr || rgb == nullptr) return fail(RTG_E_INVALID, "NULL argument");
+    if (!arena()) return fail(RTG_E_STATE, "call rtg_init first");
+    int W, H;
+    int st = resolve_size(scene->res, params, W, H);
+    if (st != RTG_OK) return st;
+    Arena &A = *arena();
+    std::lock_guard<std::mutex> l(A.lock);
+    const size_t npix = (size_t)W * H;
+    if (ensure(A.out_rgb, npix * 3) || (ids && ensure(A.out_ids, npix * 8)) || (tprim && ensure(A.out_t, npix * 4)))
+        return fail(RTG_E_NOMEM, "output buffers");
+    if (job) {
+        job->running = 1;
+        *A.cancel_host = job->cancel.load();
+    } else {
+        *A.cancel_host = 0;
+    }
+    RenderOut o{A.out_rgb.as<uint8_t>(), ids ? A.out_ids.as<int32_t>() : nullptr, tprim ? A.out_t.as<float>() : nullptr,
+                A.counters.as<unsigned long long>()};
+    RTG_HIP(hipEventRecord(A.ev0, A.stream));
+    st = render_device(scene->res, *params, W, H, o, A.stream, A);
+    RTG_HIP(hipEventRecord(A.ev1, A.stream));
+    unsigned long long cnt[4] = {0, 0, 0, 0};
+    RTG_HIP(hipMemcpyAsync(rgb, o.rgb, npix * 3, hipMemcpyDeviceToHost, A.stream));
+    if (ids) RTG_HIP(hipMemcpyAsync(ids, o.ids, npix * 8, hipMemcpyDeviceToHost, A.stream));
+    if (tprim) RTG_HIP(hipMemcpyAsync(tprim, o.tprim, npix * 4, hipMemcpyDeviceToHost, A.stream));
+    RTG_HIP(hipMemcpyAsync(cnt, o.counts, sizeof(cnt), hipMemcpyDeviceToHost, A.stream));
+    RTG_HIP(hipStreamSynchronize(A.stream));
+    if (job) job->running = 0;
+    if (st != RTG_OK) return st;
+    float ms = 0;
+    RTG_HIP(hipEventElapsedTime(&ms, A.ev0, A.ev1));
+    const bool cancelled = *A.cancel_host != 0;
+    *A.cancel_host = 0;
+    if (stats) {
+        stats->rays_primary = cnt[0];
+        stats->rays_shadow = cnt[1];
+        stats->rays_reflect = cnt[2];
+        stats->rays_refract = cnt[3];
+        stats->rays_total = cnt[0] + cnt[1] + cnt[2] + cnt[3];
+        stats->render_ms = ms;
+        stats->build_ms = scene->res.build_ms;
+        stats->retests_fp64 = 0;
+    }
+    if (cancelled) return fail(RTG_E_CANCELLED, "render cancelled");
+    return RTG_OK;
+}
+
+int rtg_render(rtg_scene_t scene, const rtg_params *params, uint8_t *rgb, int32_t *ids, float *tprim,
+               rtg_stats *stats) {
+    return render_host(scene, params, rgb, ids, tprim, stats, nullptr);
+}
+
+int rtg_render_device(rtg_scene_t scene, const rtg_params *params, uint8_t *rgb_dev, int32_t *ids_dev,
+                      float *tprim_dev, uint64_t *stats_dev, void *stream) {
+    if (scene == nullptr || rgb_dev == nullptr) return fail(RTG_E_INVALID, "NULL argument");
+    if (!arena()) return fail(RTG_E_STATE, "call rtg_init first");
+    int W, H;
+    int st = resolve_size(scene->res, params, W, H);
+    if (st != RTG_OK) return st;
+    Arena &A = *arena();
+    std::lock_guard<std::mutex> l(A.lock);
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : A.stream;
+    unsigned long long *counts =
+        stats_dev ? reinterpret_cast<unsigned long long *>(stats_dev) : A.counters.as<unsigned long long>();
+    RenderOut o{rgb_dev, ids_dev, tprim_dev, counts};
+    return render_device(scene->res, *params, W, H, o, s, A);
+}
+
+int rtg_render_async(rtg_scene_t scene, const rtg_params *params, uint8_t *rgb, int32_t *ids, float *tprim,
+                     rtg_job_t *job) {
+    if (job == nullptr || params == nullptr) return fail(RTG_E_INVALID, "NULL argument");
+    rtg_job_s *j = new rtg_job_s();
+    const rtg_params p = *params;
+    j->th = std::thread([j, scene, p, rgb, ids, tprim]() {
+        const int st = render_host(scene, &p, rgb, ids, tprim, &j->stats, j);
+        j->status = st;
+    });
+    *job = j;
+    return RTG_OK;
+}
+
+int rtg_poll(rtg_job_t job) {
+    if (job == nullptr) return fail(RTG_E_INVALID, "job is NULL");
+    return job->status.load();
+}
+
+int rtg_cancel(rtg_job_t job) {
+    if (job == nullptr) return fail(RTG_E_INVALID, "job is NULL");
+    job->cancel = 1;
+    if (job->running.load() && arena()) *arena()->cancel_host = 1;
+    return RTG_OK;
+}
+
+int rtg_job_wait(rtg_job_t job, rtg_stats *stats) {
+    if (job == nullptr) return fail(RTG_E_INVALID, "job is NULL");
+    if (job->th.joinable()) job->th.join();
+    if (stats) *stats = job->stats;
+    return job->status.load();
+}
+
+int rtg_job_free(rtg_job_t job) {
+    if (job == nullptr) return RTG_OK;
+    if (job->th.joinable()) job->th.join();
+    delete job;
+    return RTG_OK;
+}
+
+int rtg_render_batch(const rtg_request *reqs, int n, rtg_stats *stats) {
+    if (n < 0 || (n > 0 && reqs == nullptr)) return fail(RTG_E_INVALID, "bad request list");
+    rtg_stats total{};
+    for (int i = 0; i < n; ++i) {
+        rtg_stats s{};
+        int st = rtg_render(reqs[i].scene, &reqs[i].params, reqs[i].rgb, reqs[i].ids, nullptr, &s);
+        if (st != RTG_OK) return st;
+        total.rays_primary += s.rays_primary;
+        total.rays_shadow += s.rays_shadow;
+        total.rays_reflect += s.rays_reflect;
+        total.rays_refract += s.rays_refract;
+        total.rays_total += s.rays_total;
+        total.render_ms += s.render_ms;
+    }
+    if (stats) *stats = total;
+    return RTG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,212 @@
+// emul.cpp — CPU emulation of librtg's device code, for the CPU test suite.
+//
+// TEST INFRASTRUCTURE ONLY.  Compiled host-only from the same
+// __host__ __device__ headers the kernels use (rt_prep / rt_bvh / rt_shade),
+// it builds the LBVH with the device build's per-element steps (sequentially)
+// and renders pixels with the megakernel's render_sample.  It lets the CPU
+// suite check the device logic against the oracle where no GPU exists; it is
+// never part of the product and never a fallback for it.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <thread>
+#include <vector>
+
+#include "../../raytrace_amd/csrc/rt_bvh.hpp"
+#include "../../raytrace_amd/csrc/rt_prep.hpp"
+#include "../../raytrace_amd/csrc/rt_shade.hpp"
+
+using namespace rtg;
+
+namespace {
+
+struct HostScene {
+    ScenePrep prep;
+    std::vector<BvhNode> nodes;
+    std::vector<float4> prims, tri_nrm;
+    std::vector<float2> tri_uv;
+    std::vector<uint8_t> tri_bins;
+    SceneView view{};
+    int depth = 0;
+};
+
+int build(const rtg_scene_desc &d, HostScene &H) {
+    if (!prepare_scene(d, H.prep).empty()) return -1;
+    ScenePrep &P = H.prep;
+    const int64_t nt = P.n_tris;
+    const int n = (int)(nt + (int64_t)P.spheres.size());
+    std::vector<float4> prims_u((size_t)std::max(n, 1) * 3);
+    std::vector<Box6> boxes(std::max(n, 1));
+    H.tri_nrm.resize((size_t)std::max<int64_t>(nt, 1) * 3);
+    H.tri_uv.resize((size_t)std::max<int64_t>(nt, 1) * 3);
+    H.tri_bins.resize((size_t)std::max<int64_t>(nt, 1));
+    int64_t g = 0;
+    for (const int pi : P.part_order) {
+        const DevPart &DP = P.parts[pi];
+        const float4 m = P.part_mid[pi];
+        for (int t = 0; t < d.parts[pi].tri_count; ++t, ++g)
+            prim_setup_tri(P.part_tris[pi][t], P.objs[DP.obj].pos, V3{m.x, m.y, m.z}, (int)g, DP.obj, &prims_u[3 * g],
+                           &H.tri_nrm[3 * g], &H.tri_uv[3 * g], H.tri_bins[g], boxes[g]);
+    }
+    for (size_t k = 0; k < P.spheres.size(); ++k) {
+        const DevObject &o = P.objs[P.spheres[k]];
+        prim_setup_sphere(o.pos, o.radius, o.rsq, P.spheres[k], &prims_u[3 * (nt + k)], boxes[nt + k]);
+    }
+    int root = 0;
+    if (n > kLeafMax) {
+        float cb[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < 3; ++k) {
+                const float c = 0.5f * (boxes[i].lo[k] + boxes[i].hi[k]);
+                cb[k] = std::min(cb[k], c);
+                cb[3 + k] = std::max(cb[3 + k], c);
+            }
+        float cmin[3], cinv[3];
+        for (int k = 0; k < 3; ++k) {
+            cmin[k] = cb[k];
+            const float e = cb[3 + k] - cb[k];
+            cinv[k] = e > 0 ? 1.0f / e : 0.0f;
+        }
+        std::vector<uint64_t> keys(n);
+        for (int i = 0; i < n; ++i) keys[i] = morton_key(boxes[i], cmin, cinv, i);
+        std::sort(keys.begin(), keys.end());
+        std::vector<int> left(n), right(n), first(n), last(n);
+        for (int i = 0; i < n - 1; ++i) karras_node(keys.data(), n, i, left[i], right[i], first[i], last[i]);
+        // refit in post order
+        std::vector<Box6> ibox(n);
+        std::vector<std::pair<int, int>> st{{0, 0}};
+        while (!st.empty()) {
+            auto [i, state] = st.back();
+            st.pop_back();
+            if (state == 0) {
+                st.push_back({i, 1});
+                if (left[i] >= 0) st.push_back({left[i], 0});
+                if (right[i] >= 0) st.push_back({right[i], 0});
+                H.depth = std::max<int>(H.depth, (int)st.size());
+            } else {
+                auto bx = [&](int c) { return c < 0 ? boxes[(uint32_t)keys[~c]] : ibox[c]; };
+                ibox[i] = box_union(bx(left[i]), bx(right[i]));
+            }
+        }
+        std::vector<int> newidx(n, 0);
+        int n_emit = 0;
+        for (int i = 0; i < n - 1; ++i) {
+            newidx[i] = n_emit;
+            n_emit += last[i] - first[i] + 1 > kLeafMax ? 1 : 0;
+        }
+        H.nodes.resize(std::max(n_emit, 1));
+        for (int i = 0; i < n - 1; ++i) {
+            if (last[i] - first[i] + 1 <= kLeafMax) continue;
+            auto bx = [&](int c) { return c < 0 ? boxes[(uint32_t)keys[~c]] : ibox[c]; };
+            H.nodes[newidx[i]] = make_node(bx(left[i]), bx(right[i]), emit_link(left[i], first.data(), last.data(), newidx.data()),
+                                           emit_link(right[i], first.data(), last.data(), newidx.data()));
+        }
+        H.prims.resize((size_t)n * 3);
+        for (int k = 0; k < n; ++k)
+            for (int j = 0; j < 3; ++j) H.prims[3 * (size_t)k + j] = prims_u[3 * (size_t)(uint32_t)keys[k] + j];
+    } else {
+        H.prims = prims_u;
+        H.nodes.resize(1);
+        root = n > 0 ? make_leaf(0, n) : 0;
+    }
+    SceneView &v = H.view;
+    v.nodes = H.nodes.data();
+    v.prims = H.prims.data();
+    v.root = root;
+    v.n_prims = n;
+    v.objs = P.objs.data();
+    v.n_objs = (int)P.objs.size();
+    v.n_brute = (int)P.brute.size();
+    v.brute = P.brute.data();
+    v.parts = P.parts.data();
+    v.tri_nrm = H.tri_nrm.data();
+    v.tri_uv = H.tri_uv.data();
+    v.tri_part = P.tri_part.data();
+    v.tri_bins = H.tri_bins.data();
+    v.mtls = P.mtls.data();
+    v.lights = P.lights.data();
+    v.n_lights = d.n_lights;
+    v.n_texs = d.n_textures;
+    v.texs = P.texs.data();
+    v.texels = P.texels.data();
+    v.env = P.env;
+    v.cam = P.cam;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Render with the device code on the CPU.  Same output conventions as
+// rtg_render (rows bottom-up); only the listed tiles of the full-frame 64x64
+// tiling when `tiles` is given.  counts: primary, shadow, reflect, refract.
+int emul_render(const rtg_scene_desc *d, const rtg_params *p, int threads, const int32_t *tiles, int n_tiles,
+                uint8_t *rgb, int32_t *ids, float *tprim, uint64_t *counts, int *bvh_depth) {
+    HostScene H;
+    if (build(*d, H) != 0) return -1;
+    if (bvh_depth) *bvh_depth = H.depth;
+    const int W = p->width, Ht = p->height;
+    const double dp = tan((double)d->camera.fovy * 3.1415926535897932384 / 360) / (Ht / 2);
+    const int spp = p->spp > 0 ? p->spp : 1;
+    const int tx = (W + 63) / 64, ty = (Ht + 63) / 64;
+    const int total = tiles ? n_tiles : tx * ty;
+    std::atomic<int> next{0};
+    std::atomic<uint64_t> c0{0}, c1{0}, c2{0}, c3{0};
+    auto work = [&]() {
+        ArrayStack st;
+        RayCounts rc{0, 0, 0, 0};
+        for (;;) {
+            const int k = next.fetch_add(1);
+            if (k >= total) break;
+            const int tile = tiles ? tiles[k] : k;
+            const int bx = tile % tx, by = tile / tx;
+            if (by >= ty) continue;
+            if (p->tile_compat && (bx >= W / 64 || by >= Ht / 64)) continue;
+            for (int y = by * 64; y < std::min(Ht, by * 64 + 64); ++y)
+                for (int x = bx * 64; x < std::min(W, bx * 64 + 64); ++x) {
+                    V3 acc{0, 0, 0};
+                    PrimaryOut po{-1, -1, kMiss};
+                    for (int s = 0; s < spp; ++s) {
+                        float a = (float)((x - W / 2) * dp), b = (float)((y - Ht / 2) * dp);
+                        if (spp > 1) {
+                            const uint64_t h = mix64(p->seed ^ ((uint64_t)(y * W + x) * 0x9E3779B97F4A7C15ull) ^
+                                                     ((uint64_t)s * 0xC2B2AE3D27D4EB4Full));
+                            const float jx = (float)(h >> 40) * (1.0f / 16777216.0f);
+                            const float jy = (float)((h >> 16) & 0xFFFFFF) * (1.0f / 16777216.0f);
+                            const float dpf = (float)dp;
+                            a = a + (((float)(s & 1) + jx) * 0.5f) * dpf;
+                            b = b + (((float)((s >> 1) & 1) + jy) * 0.5f) * dpf;
+                        }
+                        TraceRay q;
+                        q.r.o = H.view.cam.pos;
+                        q.r.d = normalize((H.view.cam.n + H.view.cam.u * a) + H.view.cam.v * b);
+                        q.type = RAY_BASE;
+                        q.inside = 0;
+                        q.src = no_src();
+                        q.self_test = 0;
+                        PrimaryOut p0;
+                        const V3 c = render_sample(H.view, p->max_level, q, st, rc, p0);
+                        if (s == 0) po = p0;
+                        acc = acc + c;
+                    }
+                    if (spp > 1) acc = acc * (1.0f / (float)spp);
+                    const size_t pix = (size_t)y * W + x;
+                    rgb[3 * pix] = quantize(acc.x);
+                    rgb[3 * pix + 1] = quantize(acc.y);
+                    rgb[3 * pix + 2] = quantize(acc.z);
+                    if (ids) ids[2 * pix] = po.obj, ids[2 * pix + 1] = po.prim;
+                    if (tprim) tprim[pix] = po.t;
+                }
+        }
+        c0 += rc.primary, c1 += rc.shadow, c2 += rc.reflect, c3 += rc.refract;
+    };
+    std::vector<std::thread> th;
+    for (int i = 0; i < std::max(1, threads); ++i) th.emplace_back(work);
+    for (auto &t : th) t.join();
+    if (counts) counts[0] = c0, counts[1] = c1, counts[2] = c2, counts[3] = c3;
+    return 0;
+}
+
+}  // extern "C"
