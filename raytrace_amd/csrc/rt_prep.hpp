@@ -2,6 +2,7 @@
 // records of rt_scene.hpp.  Shared by the device build (rtg_bvh.hip) and the
 // CPU emulation harness of the test suite.
 #pragma once
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -52,6 +53,8 @@ inline std::string prepare_scene(const rtg_scene_desc &d, ScenePrep &P) {
         } else {
             D.bmin = prep_v3(o.vmin), D.bmax = prep_v3(o.vmax);
         }
+        if (!(std::fabs(o.radius) < 1e30f) && (o.type == RTG_OBJ_SPHERE || o.type == RTG_OBJ_BALLPLANE))
+            return "object " + std::to_string(i) + " has a non-finite or huge radius";
         if ((o.mtl < 0 || o.mtl >= d.n_materials) && o.type != RTG_OBJ_MODEL)
             return "object " + std::to_string(i) + " has no valid material";
         if (!o.show) continue;

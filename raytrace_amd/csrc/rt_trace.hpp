@@ -132,15 +132,17 @@ RTG_HD int self_testable(const SceneView &S, const Ray &r, const SrcId &src, flo
 // Centre of BallPlane sphere `k` (0..15), by the reference's float loop
 // (Basic3DObject.cpp:492-497).
 RTG_HD bool ballplane_center(const DevObject &o, int k, V3 &c) {
-    int cnt = 0;
-    for (float cx = o.radius * -6; cx < o.radius * 8; cx += o.radius * 4)
-        for (float cy = o.radius * -6; cy < o.radius * 8; cy += o.radius * 4) {
+    int cnt = 0, ix = 0;
+    for (float cx = o.radius * -6; cx < o.radius * 8 && ix < 4; cx += o.radius * 4, ++ix) {
+        int iy = 0;
+        for (float cy = o.radius * -6; cy < o.radius * 8 && iy < 4; cy += o.radius * 4, ++iy) {
             if (cnt == k) {
                 c = (o.axisx * cx + o.axisy * cy) + o.pos;
                 return true;
             }
             ++cnt;
         }
+    }
     return false;
 }
 
@@ -166,10 +168,13 @@ RTG_HD bool brute_objects(const SceneView &S, const TraceRay &q, bool anyhit, fl
                 best = Hit{t, HIT_CUBE, oi, -1, 0, 0, 0};
             }
         } else if (o.type == OBJ_BALLPLANE) {
-            int cnt = 0;
+            int cnt = 0, ix = 0;
             float run = lim;
-            for (float cx = o.radius * -6; cx < o.radius * 8; cx += o.radius * 4)
-                for (float cy = o.radius * -6; cy < o.radius * 8; cy += o.radius * 4) {
+            // 4 x 4 spheres; the iteration caps only matter for radii whose
+            // float loop would not terminate (|r| near FLT_MAX).
+            for (float cx = o.radius * -6; cx < o.radius * 8 && ix < 4; cx += o.radius * 4, ++ix) {
+                int iy = 0;
+                for (float cy = o.radius * -6; cy < o.radius * 8 && iy < 4; cy += o.radius * 4, ++iy) {
                     const int k2 = cnt++;
                     const V3 c = (o.axisx * cx + o.axisy * cy) + o.pos;
                     float t;
@@ -189,6 +194,7 @@ RTG_HD bool brute_objects(const SceneView &S, const TraceRay &q, bool anyhit, fl
                         best = Hit{t, HIT_BALL, oi, k2, 0, 0, 0};
                     }
                 }
+            }
         }
     }
     return false;
