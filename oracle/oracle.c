@@ -21,6 +21,12 @@
 #include <string.h>
 #include <time.h>
 
+/* Planted mistakes for the pinning tests (built only by the test suite). */
+#ifndef ORACLE_MUTANT
+#define ORACLE_MUTANT 0
+#endif
+#define MUT(k) (ORACLE_MUTANT == (k))
+
 #define MY_RAY_BASERAY 0x1
 #define MY_RAY_SHADOWRAY 0x2
 #define MY_RAY_REFLECTRAY 0x3
@@ -952,6 +958,7 @@ uint8_t oracle_quantize(float c) {
     if (c > 1.0f) return 255;
     if (c < 0.0f) return 0;
     float s = c * 255;
+    if (MUT(9)) s = roundf(s);
     return s == s ? (uint8_t)s : 0;
 }
 
@@ -990,6 +997,7 @@ static void render_pixel(job_t *J, octx *cx, int x, int y) {
         if (J->mode == ORACLE_REF32) {
             /* parallelRT ray generation (RayTracer.cpp:14,26-27) */
             float a = (float)(xcur * J->dp), b = (float)(ycur * J->dp);
+            if (MUT(7)) a = (float)((xcur + 0.5) * J->dp), b = (float)((ycur + 0.5) * J->dp);
             if (spp > 1) {
                 float dpf = (float)J->dp;
                 a = a + (((float)si + jx) * 0.5f) * dpf;
@@ -1182,11 +1190,7 @@ float oracle_sphere_test(const float o[4], const float d[4], const float c[4], f
 float oracle_plane_test(const float o[4], const float d[4], const float p0[4], const float n[4]) {
     RAY32 r;
     r.origin = ld4(o), r.direction = ld4(d);
-    float a = dot32(r.direction, ld4(n));
-    if ((double)fabsf(a) < 1e-6) return 1e20f;
-    float b = dot32(vsub32(r.origin, ld4(p0)), ld4(n));
-    float dis = -b / a;
-    return dis < 0 ? 1e20f : dis;
+    return plane_dist32(&r, ld4(p0), ld4(n));
 }
 float oracle_border_test(const float o[4], const float d[4], const float bmin[4], const float bmax[4], float *tmax) {
     RAY32 r;
@@ -1207,6 +1211,7 @@ void oracle_raygen(const rtg_camera *cam, int width, int height, int px, int py,
     double dp = tan((double)cam->fovy * 3.1415926535897932384 / 360) / (height / 2);
     int xcur = px - width / 2, ycur = py - height / 2;
     float a = (float)(xcur * dp), b = (float)(ycur * dp);
+    if (MUT(7)) a = (float)((xcur + 0.5) * dp), b = (float)((ycur + 0.5) * dp);
     V32 n = v432(cam->n.x, cam->n.y, cam->n.z, cam->n.w), u = v432(cam->u.x, cam->u.y, cam->u.z, cam->u.w),
         v = v432(cam->v.x, cam->v.y, cam->v.z, cam->v.w);
     V32 dir = normalize32(vadd32(vadd32(n, vscale32(u, a)), vscale32(v, b)));

@@ -38,17 +38,18 @@ class oracle_traffic(C.Structure):
     ]
 
 
-_lib = None
+_libs = {}
 _F4 = C.c_float * 4
 
 
-def load() -> C.CDLL:
-    global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(LIB_PATH):
-        raise FileNotFoundError(f"{LIB_PATH} is not built (make -C oracle)")
-    lib = C.CDLL(LIB_PATH)
+def load(path: str | None = None) -> C.CDLL:
+    """Load the oracle library (default build, or another build at `path`)."""
+    path = path or LIB_PATH
+    if path in _libs:
+        return _libs[path]
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is not built (make -C oracle)")
+    lib = C.CDLL(path)
     P = C.c_void_p
     lib.oracle_scene_create.argtypes = [P, C.POINTER(P)]
     lib.oracle_scene_create.restype = C.c_int
@@ -75,7 +76,7 @@ def load() -> C.CDLL:
     lib.oracle_raygen.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, _F4, _F4]
     lib.oracle_quantize.argtypes = [C.c_float]
     lib.oracle_quantize.restype = C.c_uint8
-    _lib = lib
+    _libs[path] = lib
     return lib
 
 
@@ -87,8 +88,8 @@ def f4(v) -> "C.Array":
 class OracleScene:
     """Oracle preparation (Model::RTPrepare + canonical BVH) of a scene descriptor."""
 
-    def __init__(self, desc_ptr: int):
-        self.lib = load()
+    def __init__(self, desc_ptr: int, lib: C.CDLL | None = None):
+        self.lib = lib or load()
         h = C.c_void_p()
         rc = self.lib.oracle_scene_create(C.c_void_p(desc_ptr), C.byref(h))
         if rc != 0:

@@ -40,6 +40,61 @@ class rtg_material(C.Structure):
     ]
 
 
+class rtg_texture(C.Structure):
+    _fields_ = [("w", C.c_int32), ("h", C.c_int32), ("bgr", C.POINTER(C.c_uint8))]
+
+
+class rtg_object(C.Structure):
+    _fields_ = [
+        ("type", C.c_int32), ("show", C.c_int32), ("mtl", C.c_int32), ("tex", C.c_int32),
+        ("position", rtg_vec4), ("radius", C.c_float), ("radius_sqr", C.c_float),
+        ("part_begin", C.c_int32), ("part_count", C.c_int32),
+        ("normal", rtg_vec4), ("axisx", rtg_vec4), ("axisy", rtg_vec4), ("vmin", rtg_vec4), ("vmax", rtg_vec4),
+    ]
+
+
+class rtg_triangle(C.Structure):
+    _fields_ = [("p", rtg_vec4 * 3), ("n", rtg_vec4 * 3), ("uv", C.c_float * 6), ("pad", C.c_float * 2)]
+
+
+class rtg_part(C.Structure):
+    _fields_ = [
+        ("tris", C.POINTER(rtg_triangle)), ("tri_count", C.c_int32), ("mtl", C.c_int32),
+        ("bmin", rtg_vec4), ("bmax", rtg_vec4),
+    ]
+
+
+class rtg_light(C.Structure):
+    _fields_ = [
+        ("type", C.c_int32), ("on", C.c_int32), ("pad", C.c_int32 * 2),
+        ("position", rtg_vec4), ("ambient", rtg_vec4), ("diffuse", rtg_vec4), ("specular", rtg_vec4),
+        ("attenuation", rtg_vec4),
+    ]
+
+
+class rtg_camera(C.Structure):
+    _fields_ = [
+        ("position", rtg_vec4), ("u", rtg_vec4), ("v", rtg_vec4), ("n", rtg_vec4),
+        ("width", C.c_int32), ("height", C.c_int32), ("fovy", C.c_float), ("zNear", C.c_float),
+        ("zFar", C.c_float), ("pad", C.c_float * 3),
+    ]
+
+
+class rtg_scene_desc(C.Structure):
+    _fields_ = [
+        ("objects", C.POINTER(rtg_object)), ("n_objects", C.c_int32), ("n_parts", C.c_int32),
+        ("parts", C.POINTER(rtg_part)), ("n_tris", C.c_int64), ("materials", C.POINTER(rtg_material)),
+        ("n_materials", C.c_int32), ("n_textures", C.c_int32), ("textures", C.POINTER(rtg_texture)),
+        ("lights", C.POINTER(rtg_light)), ("n_lights", C.c_int32), ("pad", C.c_int32),
+        ("env_light", rtg_vec4), ("camera", rtg_camera),
+    ]
+
+
+def desc_of(ptr: int) -> rtg_scene_desc:
+    """View of a flattened scene descriptor (valid while its HostScene lives)."""
+    return rtg_scene_desc.from_address(ptr)
+
+
 class rtg_params(C.Structure):
     _fields_ = [
         ("mode", C.c_int32), ("max_level", C.c_int32), ("width", C.c_int32), ("height", C.c_int32),

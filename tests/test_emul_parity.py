@@ -1,0 +1,77 @@
+"""The device code run on the CPU (tests/cpu_emul) against the REF32 oracle.
+
+The kernels' traversal, self-skip emulation, shading and ray-tree
+composition are __host__ __device__ code; compiled for the host they must
+reproduce the oracle's reference algorithm (object scan over octant-binned
+clTri copies) bit for bit, up to grazing cases of the reference's culling.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import raytrace_amd as rt
+from conftest import compare_frames, emul_render, sample_tiles, tile_mask
+
+
+def _run(hs, p, tiles=None):
+    from oracle import pyoracle as po
+    o = po.OracleScene(hs.desc_ptr).render(p, po.ORACLE_REF32, tiles=tiles)
+    e = emul_render(hs.desc_ptr, p, tiles=tiles)
+    return o, e
+
+
+def _assert_parity(o, e, region=None, exact=True):
+    s = compare_frames(e, o, region)
+    assert s["ids_mismatch_outside_band"] == 0, s
+    assert s["ids_match"] >= 0.9999, s
+    assert s["rgb_over_1"] == 0, s
+    if exact:
+        assert s["rgb_max"] == 0 and s["t_exact"] == 1.0, s
+    return s
+
+
+def test_workload1_exact():
+    hs = rt.HostScene.workload(1)
+    p = rt.params(640, 480, max_level=3)
+    o, e = _run(hs, p)
+    _assert_parity(o, e)
+    st = o["stats"]
+    assert list(e["counts"]) == [st.rays_primary, st.rays_shadow, st.rays_reflect, st.rays_refract]
+
+
+def test_workload2_full_mesh_reduced_frame():
+    hs = rt.HostScene.workload(2, width=320, height=180)
+    p = rt.params(320, 180, max_level=4)
+    o, e = _run(hs, p)
+    _assert_parity(o, e)
+    assert e["depth"] <= 62
+
+
+@pytest.mark.parametrize("index", [0, 1, 2, 3])
+def test_batch_scenes(index):
+    """Triangle soups (even) and height fields (odd) of the batch workload."""
+    hs = rt.HostScene.workload(3, index=index, width=192, height=128, detail=4000)
+    p = rt.params(192, 128, max_level=hs.info.max_level)
+    o, e = _run(hs, p)
+    _assert_parity(o, e, exact=False)
+
+
+def test_refraction_spp4_terrain_scene():
+    """Stress-workload features at reduced size: refraction through sphere
+    meshes, 16 lights, 4 jittered samples, depth 8."""
+    hs = rt.HostScene.workload(5, width=256, height=144, detail=32, n_spheres=24, n_lights=16)
+    p = rt.params(256, 144, max_level=8, spp=4, seed=99)
+    o, e = _run(hs, p)
+    s = _assert_parity(o, e, exact=False)
+    assert o["stats"].rays_refract > 0
+    assert list(e["counts"]) == [o["stats"].rays_primary, o["stats"].rays_shadow, o["stats"].rays_reflect,
+                                 o["stats"].rays_refract]
+
+
+def test_tile_compat_leaves_partial_tiles():
+    hs = rt.HostScene.workload(1, width=200, height=150)
+    p = rt.params(200, 150, max_level=3, tile_compat=1)
+    o, e = _run(hs, p)
+    assert (e["rgb"][128:] == 127).all() and (e["rgb"][:, 192:] == 127).all()
+    assert (e["rgb"] == o["rgb"]).all()

@@ -1,0 +1,157 @@
+"""Host side of the drop-in API (raytrace_amd/host): the scene values the
+reference's Scene / Light / Plane / Model code produces, checked against
+closed forms.  No GPU needed."""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+
+import numpy as np
+import pytest
+
+import raytrace_amd as rt
+from raytrace_amd import _capi
+
+
+def desc(hs):
+    return _capi.desc_of(hs.desc_ptr)
+
+
+def v(a):
+    return np.array([a.x, a.y, a.z], np.float64)
+
+
+def test_default_plane_frame():
+    """Plane() = rotate((0,36,0)) (Basic3DObject.cpp:304-330): normal ~ +y through
+    the origin, axes orthonormal and in the plane, 4x4 grey checker texture."""
+    hs = rt.HostScene.new()
+    hs.add_plane()
+    d = desc(hs)
+    o = d.objects[0]
+    assert o.type == 4
+    n, ax, ay = v(o.normal), v(o.axisx), v(o.axisy)
+    assert np.allclose(n, [0, 1, 0], atol=1e-7) and np.allclose(v(o.position), 0)
+    assert abs(np.linalg.norm(ax) - 1) < 1e-6 and abs(np.linalg.norm(ay) - 1) < 1e-6
+    assert abs(ax @ n) < 1e-6 and abs(ay @ n) < 1e-6 and abs(ax @ ay) < 1e-6
+    t = d.textures[o.tex]
+    tex = np.ctypeslib.as_array(t.bgr, shape=(t.w * t.h * 3,)).reshape(4, 4, 3)
+    assert (t.w, t.h) == (4, 4)
+    for a in range(4):
+        for b in range(4):
+            assert (tex[a, b] == (255 if (a & 2) == (b & 2) else 127)).all()
+    assert d.materials[o.mtl].reflect == np.float32(0.6)  # AddPlane material
+
+
+def test_add_light_normalises_and_scales():
+    """AddLight (Scene.cpp:83-97): components divided by their sum, then
+    SetLumi(atte.w) scales them; attenuation keeps (x, y, z)."""
+    hs = rt.HostScene.new()
+    hs.add_light(2, (0.15, 0.55, 0.3), (0, 0, 1, 256))
+    L = desc(hs).lights[0]
+    assert L.type == 2 and L.on == 1
+    assert np.allclose([L.ambient.x, L.diffuse.x, L.specular.x], [0.15 * 256, 0.55 * 256, 0.3 * 256], rtol=1e-6)
+    assert (L.attenuation.x, L.attenuation.y, L.attenuation.z, L.attenuation.w) == (0, 0, 1, 256)
+    assert L.position.w == 1.0  # point light
+    # default position: rangy 90, rangz 0, rdis 16 -> (0, ~0, 16)
+    assert np.allclose(v(L.position), [0, 0, 16], atol=1e-5)
+
+
+def test_light_move_spherical():
+    """Light::move (3DElement.cpp:533-545) as main.cpp:166-167 drives it:
+    angles (90 - 45, 0 + 45) at distance 16."""
+    hs = rt.HostScene.new()
+    hs.add_light(1, (0.1, 0.45, 0.45))
+    hs.move_pos(1, 0, (-45, 45, 0))
+    L = desc(hs).lights[0]
+    s = math.sin(math.radians(45))
+    assert np.allclose(v(L.position), [16 * s * s, 16 * math.cos(math.radians(45)), 16 * s * s], rtol=1e-6)
+    assert L.position.w == 0.0  # parallel light
+
+
+def test_light_cap_is_eight():
+    hs = rt.HostScene.new()
+    idx = [hs.add_light(2, (1, 1, 1)) for _ in range(9)]
+    assert idx[:8] == list(range(8)) and idx[8] == 255
+
+
+def test_add_sphere_pose_and_material():
+    hs = rt.HostScene.new()
+    i = hs.add_sphere(1.25)
+    o = desc(hs).objects[i]
+    assert o.type == 1 and (o.position.x, o.position.y, o.position.z) == (0, 1.25, 0)
+    assert o.radius == np.float32(1.25) and o.radius_sqr == np.float32(1.25) * np.float32(1.25)
+    m = desc(hs).materials[o.mtl]
+    assert m.reflect == np.float32(0.35) and m.shiness == 100  # "bas-sphere"
+
+
+def test_switch_hides_object():
+    hs = rt.HostScene.new()
+    hs.add_sphere(1.0)
+    assert hs.switch(2, 0, False) == 1  # returns the previous state
+    assert desc(hs).objects[0].show == 0
+    assert hs.switch(2 | 0x80, 0, True) == 0  # toggle
+    assert desc(hs).objects[0].show == 1
+
+
+def test_workload2_icosphere_geometry():
+    """Workload 2: 20 parts x F^2 triangles, extent-8 rescale, unit normals."""
+    hs = rt.HostScene.workload(2, detail=10)
+    d = desc(hs)
+    assert hs.info.n_tris == 2000 and d.n_parts == 20
+    o = d.objects[0]
+    ext = [o.vmax.x - o.vmin.x, o.vmax.y - o.vmin.y, o.vmax.z - o.vmin.z]
+    assert abs(max(ext) - 8.0) < 1e-5  # Model.cpp:185-196
+    pts, nrm = [], []
+    for p in range(d.n_parts):
+        part = d.parts[p]
+        assert part.tri_count == 100
+        for t in range(part.tri_count):
+            tri = part.tris[t]
+            pts += [v(tri.p[k]) for k in range(3)]
+            nrm += [v(tri.n[k]) for k in range(3)]
+    pts, nrm = np.array(pts), np.array(nrm)
+    r = np.linalg.norm(pts, axis=1)
+    assert 3.0 < r.min() and r.max() < 4.5  # radius 1 +- 0.1 noise, scaled to an extent of 8
+    lo, hi = v(o.vmin), v(o.vmax)
+    assert (pts >= lo - 1e-6).all() and (pts <= hi + 1e-6).all()
+    assert np.allclose(np.linalg.norm(nrm, axis=1), 1, atol=1e-6)
+    assert ((nrm * pts).sum(1) > 0).mean() > 0.99  # outward
+
+
+def test_obj_loader_quads_parts_rescale(tmp_path):
+    """OBJ/MTL reading (Model.cpp:14-281): quads split along 0-2, one part per
+    usemtl, material by name, extent-8 rescale, vn normals kept."""
+    obj = tmp_path / "m.obj"
+    mtl = tmp_path / "m.mtl"
+    mtl.write_text("newmtl red\nKd 1 0 0\nKa 0.1 0 0\nNs 20\nnewmtl blue\nKd 0 0 1\n")
+    obj.write_text("v 0 0 0\nv 2 0 0\nv 2 2 0\nv 0 2 0\nv 0 0 1\nvn 0 0 1\n"
+                   "usemtl red\nf 1//1 2//1 3//1 4//1\nusemtl blue\nf 1//1 2//1 5//1\n")
+    hs = rt.HostScene.new()
+    i = hs.add_model_obj(str(obj), str(mtl))
+    d = desc(hs)
+    o = d.objects[i]
+    assert o.type == 3 and o.part_count == 2
+    p0, p1 = d.parts[o.part_begin], d.parts[o.part_begin + 1]
+    assert (p0.tri_count, p1.tri_count) == (2, 1)
+    # extent 2 -> scale 4
+    t = p0.tris[1]
+    assert [v(t.p[k]).tolist() for k in range(3)] == [[0, 0, 0], [8, 8, 0], [0, 8, 0]]
+    assert d.materials[p0.mtl].diffuse.x == 1.0 and d.materials[p0.mtl].shiness == 20
+    assert d.materials[p1.mtl].diffuse.z == 1.0
+    # newmtl does not reset the running material: blue keeps Ka and Ns of red
+    assert d.materials[p1.mtl].ambient.x == np.float32(0.1) and d.materials[p1.mtl].shiness == 20
+
+
+def test_missing_obj_is_an_empty_model(tmp_path):
+    hs = rt.HostScene.new()
+    i = hs.add_model_obj(str(tmp_path / "none.obj"), str(tmp_path / "none.mtl"))
+    assert desc(hs).objects[i].type == 3
+
+
+def test_scene_descriptor_is_deterministic():
+    a = desc(rt.HostScene.workload(1))
+    b = desc(rt.HostScene.workload(1))
+    for k in range(a.n_objects):
+        assert bytes(a.objects[k]) == bytes(b.objects[k])
+    assert bytes(a.lights[0]) == bytes(b.lights[0])
