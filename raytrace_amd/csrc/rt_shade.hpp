@@ -137,28 +137,34 @@ RTG_HD LightTerm light_term(const DevLight &L, V3 P) {
     return lt;
 }
 
-// Local colour vc (kd...+ka...) + specular (RayTracer.cpp:469-546).  `blocked`
+// Local colour vc (kd...+ka...) + specular (RayTracer.cpp:469-546) at hit
+// point P with normal N, texel colour vc, incoming direction D.  `blocked`
 // answers the shadow test of light li for the given shadow ray.
 template <class Blocked>
-RTG_HD V3 local_color(const SceneView &S, V3 D, const HitRec &hr, Blocked &&blocked) {
-    const DevMaterial &m = S.mtls[hr.mtl];
-    const V3 vc = texel(S, hr.tex, hr.tu, hr.tv);
+RTG_HD V3 shade_local(const SceneView &S, V3 D, V3 P, V3 N, int mtl, V3 vc, Blocked &&blocked) {
+    const DevMaterial &m = S.mtls[mtl];
     V3 vd = V3{0, 0, 0}, vs = V3{0, 0, 0};
     V3 va = mixmul(m.ka, S.env);
     for (int li = 0; li < S.n_lights; ++li) {
         const DevLight &L = S.lights[li];
         if (!L.on) continue;
-        const LightTerm lt = light_term(L, hr.P);
+        const LightTerm lt = light_term(L, P);
         va = va + mixmul(m.ka, lt.la);
         if (blocked(li, lt)) continue;
-        float n_n = dot(hr.N, lt.p2l);
+        float n_n = dot(N, lt.p2l);
         if (n_n > 0) vd = vd + mixmul(m.kd, lt.ld) * n_n;
         const V3 h = normalize(lt.p2l - D);
-        n_n = dot(hr.N, h);
+        n_n = dot(N, h);
         if (n_n > 0) vs = vs + mixmul(m.ks, lt.ls) * powf(n_n, m.shiness);
     }
     return mixmul(vc, vd + va) + vs;
 }
+
+template <class Blocked>
+RTG_HD V3 local_color(const SceneView &S, V3 D, const HitRec &hr, Blocked &&blocked) {
+    return shade_local(S, D, hr.P, hr.N, hr.mtl, texel(S, hr.tex, hr.tu, hr.tv), blocked);
+}
+
 
 // Child ray of a hit.
 RTG_HD TraceRay reflect_ray(const TraceRay &q, const HitRec &hr) {
@@ -193,6 +199,25 @@ RTG_HD bool refract_ray(const TraceRay &q, float mtlrfr, const HitRec &hr, Trace
 RTG_HD V3 beer(const V3 &kd, float child_alpha) {
     const V3 e = (kd * 0.15f) * (-child_alpha);
     return V3{expf(e.x), expf(e.y), expf(e.z)};
+}
+
+// Ray-tree node composition (RayTracer.cpp:546-595) for the wavefront
+// engine's bottom-up pass: reflection child c_refl, refraction child c_refr
+// with hit distance refr_alpha; `tir` = total internal reflection.  Same
+// operations, in the same order, as the megakernel's frame stages.
+RTG_HD V3 compose(V3 c, float rl, float rr, bool tir, bool inside, V3 kd, V3 c_refl, V3 c_refr, float refr_alpha) {
+    if (rl > 0.01f) {
+        c = c * (1.0f - rl);
+        c = c + c_refl * rl;
+    }
+    if (rr > 0.01f) {
+        c = c * (1.0f - rr);
+        if (!tir) {
+            const V3 w = inside ? beer(kd, refr_alpha) : V3{1.0f, 1.0f, 1.0f};
+            c = c + mixmul(c_refr, w) * rr;
+        }
+    }
+    return c;
 }
 
 struct RayCounts {

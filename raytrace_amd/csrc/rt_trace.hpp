@@ -63,15 +63,20 @@ RTG_HD bool slab(V3 o, V3 inv, float lx, float hx, float ly, float hy, float lz,
 struct ArrayStack {
     int d[128];
     int sp = 0;
+    RTG_HD void reset() { sp = 0; }
     RTG_HD void push(int v) { d[sp++] = v; }
     RTG_HD int pop() { return d[--sp]; }
     RTG_HD bool empty() const { return sp == 0; }
 };
 
 // Visit the primitives of every leaf the segment [0, tmax] reaches, nearer
-// child first.  visit(prim_index, tmax&) returns true to stop.
+// child first.  visit(prim_index, tmax&) returns true to stop.  The stack is
+// emptied on entry: an any-hit walk that stops early leaves entries behind,
+// and the stack only holds one entry per level of the current path (a tree
+// of depth < kStackMax) when every walk starts from an empty stack.
 template <class Stack, class Visit>
 RTG_HD void bvh_walk(const SceneView &S, const Ray &r, float &tmax, Stack &st, Visit &&visit) {
+    st.reset();
     if (S.n_prims == 0) return;
     const V3 inv = safe_inv(r.d);
     int node = S.root;

@@ -234,6 +234,7 @@ int rtg_scene_update(rtg_scene_t scene, const rtg_scene_desc *desc) {
     R.build_ms = fresh.build_ms;
     R.bytes = fresh.bytes;
     R.n_nodes = fresh.n_nodes;
+    R.lights_on = fresh.lights_on;
     R.geom_hash = geometry_hash(*desc);
     return RTG_OK;
 }

@@ -293,6 +293,9 @@ int scene_build(SceneRes &S, const rtg_scene_desc &d, hipStream_t stream) {
     (void)hipEventDestroy(e1);
     S.build_ms = ms;
     S.n_nodes = n_emit;
+    S.lights_on = 0;
+    for (int i = 0; i < d.n_lights; ++i)
+        if (d.lights[i].on) S.lights_on |= 1u << i;
 
     SceneView &v = S.view;
     v = SceneView{};

@@ -45,6 +45,7 @@ struct SceneRes {
     double build_ms = 0.0;
     size_t bytes = 0;
     int n_nodes = 0;
+    uint32_t lights_on = 0;  // bit i: light i is switched on
 };
 
 // Per-device render state: stream, spill stacks, tables, counters, outputs.
@@ -54,10 +55,19 @@ struct Arena {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBuf spill, tables, counters, out_rgb, out_ids, out_t;
+    uint64_t tables_key = 0;
     int *cancel_host = nullptr;  // pinned, mapped
     int *cancel_dev = nullptr;
+    // wavefront engine: per-level queues and records, grown on demand
+    static constexpr int kLevels = RTG_MAX_LEVEL + 1;
+    DevBuf w_rays[kLevels], w_hits[kLevels], w_nodes[kLevels], w_masks[kLevels], w_slots[kLevels],
+        w_shadow[kLevels], w_samples, w_qcount;
+    unsigned *qcount_host = nullptr;  // pinned
     std::mutex lock;  // one render at a time on the arena
 };
+
+int upload_raygen_tables(const SceneRes &S, int W, int H, hipStream_t stream, Arena &A, double &dp);
+int persistent_blocks(const Arena &A);
 
 Arena *arena();
 int ensure(DevBuf &b, size_t bytes);

@@ -1,38 +1,13 @@
 // rtg_mega.hip — the one-launch-per-frame megakernel.
 //
-// Persistent grid (a few blocks per CU), 128 threads per block = two waves,
-// each wave an 8x8 pixel block.  Pixel blocks are split into 8 contiguous
-// image regions, one per XCD (block b runs on XCD b % 8), so each XCD's L2
-// caches the part of the BVH its region touches.  Traversal stacks live in
-// LDS (32 entries per thread) and spill to a per-thread global area; a Karras
-// tree over 62-bit unique keys is at most 63 levels deep, so 64 entries bound
-// the stack.
+// Persistent grid (8 blocks per CU), 128 threads per block = two waves, each
+// wave an 8x8 pixel block; pixel blocks are split over the XCDs by
+// xcd_chunks so each XCD's L2 caches the part of the BVH its image region
+// touches.  Each thread evaluates its pixel's whole ray tree (render_sample).
+#include "rt_devstack.hpp"
 #include "rtg_internal.hpp"
 
 namespace rtg {
-
-constexpr int kMegaBlock = 128;
-constexpr int kLdsStack = 32;
-constexpr int kStackMax = 64;
-
-struct DevStack {
-    int *lds;
-    int *spill;
-    int gstride;
-    int sp;
-    __device__ void push(int v) {
-        if (sp < kLdsStack)
-            lds[sp * kMegaBlock] = v;
-        else
-            spill[(sp - kLdsStack) * gstride] = v;
-        ++sp;
-    }
-    __device__ int pop() {
-        --sp;
-        return sp < kLdsStack ? lds[sp * kMegaBlock] : spill[(sp - kLdsStack) * gstride];
-    }
-    __device__ bool empty() const { return sp == 0; }
-};
 
 struct MegaArgs {
     int W, H, spp, max_level, tile_compat;
@@ -48,21 +23,37 @@ struct MegaArgs {
     int *spill;
 };
 
-__device__ inline uint32_t wave_sum(uint32_t v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+// Primary ray of sample s of pixel (px, py): the reference's pixel-corner ray
+// for spp 1 (RayTracer.cpp:26-27), 2x2 jittered strata otherwise.
+__device__ inline TraceRay primary_ray(const SceneView &S, const float *ax, const float *by, int W, int px, int py,
+                                       int s, int spp, float dpf, uint64_t seed) {
+    float a = ax[px], b = by[py];
+    if (spp > 1) {
+        const uint64_t h =
+            mix64(seed ^ ((uint64_t)(py * W + px) * 0x9E3779B97F4A7C15ull) ^ ((uint64_t)s * 0xC2B2AE3D27D4EB4Full));
+        const float jx = (float)(h >> 40) * (1.0f / 16777216.0f);
+        const float jy = (float)((h >> 16) & 0xFFFFFF) * (1.0f / 16777216.0f);
+        a = a + (((float)(s & 1) + jx) * 0.5f) * dpf;
+        b = b + (((float)((s >> 1) & 1) + jy) * 0.5f) * dpf;
+    }
+    TraceRay q;
+    q.r.o = S.cam.pos;
+    q.r.d = normalize((S.cam.n + S.cam.u * a) + S.cam.v * b);
+    q.type = RAY_BASE;
+    q.inside = 0;
+    q.src = no_src();
+    q.self_test = 0;
+    return q;
 }
 
-__global__ __launch_bounds__(kMegaBlock) void k_megakernel(SceneView S, MegaArgs A) {
-    __shared__ int lds[kLdsStack * kMegaBlock];
+__global__ __launch_bounds__(kBlockThreads) void k_megakernel(SceneView S, MegaArgs A) {
+    __shared__ int lds[kLdsStack * kBlockThreads];
     const int tid = threadIdx.x;
-    DevStack st{lds + tid, A.spill + (size_t)blockIdx.x * kMegaBlock + tid, (int)gridDim.x * kMegaBlock, 0};
+    DevStack st = make_stack(lds, A.spill);
     RayCounts rc{0, 0, 0, 0};
-    const int G = gridDim.x, xcd = blockIdx.x & 7, local = blockIdx.x >> 3, per_xcd = G >> 3;
-    const int region = (A.nblocks + 7) >> 3;
-    const int start = xcd * region, end = min(A.nblocks, start + region);
+    const ChunkRange cr = xcd_chunks(A.nblocks);
     const int tw = (A.W >> 6) << 6, th = (A.H >> 6) << 6;
-    for (int pb = start + local; pb < end; pb += per_xcd) {
+    for (int pb = cr.first; pb < cr.end; pb += cr.step) {
         if (__builtin_nontemporal_load(A.cancel) != 0) break;
         const int bx = pb % A.bw, by = pb / A.bw;
         const int px = bx * 16 + ((tid >> 6) << 3) + (tid & 7), py = by * 8 + ((tid & 63) >> 3);
@@ -71,22 +62,7 @@ __global__ __launch_bounds__(kMegaBlock) void k_megakernel(SceneView S, MegaArgs
         V3 acc = V3{0, 0, 0};
         PrimaryOut po{-1, -1, kMiss};
         for (int s = 0; s < A.spp; ++s) {
-            float a = A.ax[px], b = A.by[py];
-            if (A.spp > 1) {
-                const uint64_t h = mix64(A.seed ^ ((uint64_t)(py * A.W + px) * 0x9E3779B97F4A7C15ull) ^
-                                         ((uint64_t)s * 0xC2B2AE3D27D4EB4Full));
-                const float jx = (float)(h >> 40) * (1.0f / 16777216.0f);
-                const float jy = (float)((h >> 16) & 0xFFFFFF) * (1.0f / 16777216.0f);
-                a = a + (((float)(s & 1) + jx) * 0.5f) * A.dpf;
-                b = b + (((float)((s >> 1) & 1) + jy) * 0.5f) * A.dpf;
-            }
-            TraceRay q;
-            q.r.o = S.cam.pos;
-            q.r.d = normalize((S.cam.n + S.cam.u * a) + S.cam.v * b);
-            q.type = RAY_BASE;
-            q.inside = 0;
-            q.src = no_src();
-            q.self_test = 0;
+            const TraceRay q = primary_ray(S, A.ax, A.by, A.W, px, py, s, A.spp, A.dpf, A.seed);
             PrimaryOut p0;
             const V3 c = render_sample(S, A.max_level, q, st, rc, p0);
             if (s == 0) po = p0;
@@ -103,8 +79,8 @@ __global__ __launch_bounds__(kMegaBlock) void k_megakernel(SceneView S, MegaArgs
         }
         if (A.tprim) A.tprim[pix] = po.t;
     }
-    const uint32_t c0 = wave_sum(rc.primary), c1 = wave_sum(rc.shadow), c2 = wave_sum(rc.reflect),
-                   c3 = wave_sum(rc.refract);
+    const uint32_t c0 = wave_sum_u32(rc.primary), c1 = wave_sum_u32(rc.shadow), c2 = wave_sum_u32(rc.reflect),
+                   c3 = wave_sum_u32(rc.refract);
     if ((tid & 63) == 0) {
         atomicAdd(&A.counts[0], (unsigned long long)c0);
         atomicAdd(&A.counts[1], (unsigned long long)c1);
@@ -113,18 +89,33 @@ __global__ __launch_bounds__(kMegaBlock) void k_megakernel(SceneView S, MegaArgs
     }
 }
 
-int launch_megakernel(const SceneRes &S, const rtg_params &p, int W, int H, const RenderOut &o, hipStream_t stream,
-                      Arena &A) {
-    // ray-generation tables: (float)(xcur * dp), (float)(ycur * dp) (RayTracer.cpp:14,26)
-    const double dp = tan((double)S.camera.fovy * 3.1415926535897932384 / 360) / (H / 2);
+// Ray-generation tables (float)(xcur * dp), (float)(ycur * dp) with
+// dp = tan(fovy * PI / 360) / (H / 2) in double (RayTracer.cpp:14,26).
+// Uploaded once per (W, H, fovy); later frames reuse them.
+int upload_raygen_tables(const SceneRes &S, int W, int H, hipStream_t stream, Arena &A, double &dp) {
+    dp = tan((double)S.camera.fovy * 3.1415926535897932384 / 360) / (H / 2);
+    const uint32_t fb = __builtin_bit_cast(uint32_t, S.camera.fovy);
+    const uint64_t key = (uint64_t)W | ((uint64_t)H << 16) | ((uint64_t)fb << 32);
+    if (A.tables.p != nullptr && A.tables_key == key) return RTG_OK;
     std::vector<float> tab((size_t)W + H);
     for (int x = 0; x < W; ++x) tab[x] = (float)((x - W / 2) * dp);
     for (int y = 0; y < H; ++y) tab[W + y] = (float)((y - H / 2) * dp);
     if (ensure(A.tables, tab.size() * sizeof(float))) return fail(RTG_E_NOMEM, "tables");
-    RTG_HIP(hipMemcpyAsync(A.tables.p, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice, stream));
-    const int blocks = std::max(8, (A.cus > 0 ? A.cus : 256) * 8);
-    const size_t spill = (size_t)blocks * kMegaBlock * (kStackMax - kLdsStack) * sizeof(int);
-    if (ensure(A.spill, spill)) return fail(RTG_E_NOMEM, "spill");
+    RTG_HIP(hipStreamSynchronize(stream));  // earlier frames may still read the old tables
+    RTG_HIP(hipMemcpy(A.tables.p, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
+    A.tables_key = key;
+    return RTG_OK;
+}
+
+int persistent_blocks(const Arena &A) { return std::max(8, (A.cus > 0 ? A.cus : 256) * 8); }
+
+int launch_megakernel(const SceneRes &S, const rtg_params &p, int W, int H, const RenderOut &o, hipStream_t stream,
+                      Arena &A) {
+    double dp;
+    int st = upload_raygen_tables(S, W, H, stream, A, dp);
+    if (st != RTG_OK) return st;
+    const int blocks = persistent_blocks(A);
+    if (ensure(A.spill, spill_ints(blocks) * sizeof(int))) return fail(RTG_E_NOMEM, "spill");
     MegaArgs a;
     a.W = W, a.H = H, a.spp = p.spp > 0 ? p.spp : 1, a.max_level = p.max_level, a.tile_compat = p.tile_compat;
     a.bw = (W + 15) / 16;
@@ -137,7 +128,7 @@ int launch_megakernel(const SceneRes &S, const rtg_params &p, int W, int H, cons
     a.counts = o.counts;
     a.cancel = A.cancel_dev;
     a.spill = A.spill.as<int>();
-    k_megakernel<<<blocks, kMegaBlock, 0, stream>>>(S.view, a);
+    k_megakernel<<<blocks, kBlockThreads, 0, stream>>>(S.view, a);
     RTG_HIP(hipGetLastError());
     return RTG_OK;
 }

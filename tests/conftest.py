@@ -37,6 +37,7 @@ def emul_lib():
         lib = C.CDLL(EMUL_PATH)
         lib.emul_render.argtypes = [C.c_void_p] * 2 + [C.c_int, C.c_void_p, C.c_int] + [C.c_void_p] * 5
         lib.emul_render.restype = C.c_int
+        lib.emul_max_stack.restype = C.c_int
         _emul = lib
     return _emul
 
@@ -53,7 +54,8 @@ def emul_render(desc_ptr: int, p, threads: int = 8, tiles=None) -> dict:
     rc = emul_lib().emul_render(C.c_void_p(desc_ptr), C.byref(p), threads, ptr(tl), 0 if tl is None else len(tl),
                                 ptr(rgb), ptr(ids), ptr(t), ptr(cnt), C.byref(depth))
     assert rc == 0
-    return {"rgb": rgb, "ids": ids, "t": t, "counts": cnt, "depth": depth.value}
+    return {"rgb": rgb, "ids": ids, "t": t, "counts": cnt, "depth": depth.value,
+            "max_stack": emul_lib().emul_max_stack()}
 
 
 def compare_frames(test: dict, ref: dict, region=None) -> dict:

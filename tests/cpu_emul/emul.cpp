@@ -21,6 +21,23 @@ using namespace rtg;
 
 namespace {
 
+// Traversal stack that records its high-water mark (the device stack holds
+// kLdsStack entries in LDS and kStackMax in total).
+std::atomic<int> g_max_sp{0};
+struct TrackStack {
+    int d[256];
+    int sp = 0;
+    void reset() { sp = 0; }
+    void push(int v) {
+        d[sp++] = v;
+        int m = g_max_sp.load();
+        while (sp > m && !g_max_sp.compare_exchange_weak(m, sp)) {
+        }
+    }
+    int pop() { return d[--sp]; }
+    bool empty() const { return sp == 0; }
+};
+
 struct HostScene {
     ScenePrep prep;
     std::vector<BvhNode> nodes;
@@ -75,15 +92,17 @@ int build(const rtg_scene_desc &d, HostScene &H) {
         for (int i = 0; i < n - 1; ++i) karras_node(keys.data(), n, i, left[i], right[i], first[i], last[i]);
         // refit in post order
         std::vector<Box6> ibox(n);
+        std::vector<int> level(n, 0);
         std::vector<std::pair<int, int>> st{{0, 0}};
+        level[0] = 1;
         while (!st.empty()) {
             auto [i, state] = st.back();
             st.pop_back();
             if (state == 0) {
                 st.push_back({i, 1});
-                if (left[i] >= 0) st.push_back({left[i], 0});
-                if (right[i] >= 0) st.push_back({right[i], 0});
-                H.depth = std::max<int>(H.depth, (int)st.size());
+                if (left[i] >= 0) st.push_back({left[i], 0}), level[left[i]] = level[i] + 1;
+                if (right[i] >= 0) st.push_back({right[i], 0}), level[right[i]] = level[i] + 1;
+                H.depth = std::max<int>(H.depth, level[i] + 1);  // + the leaf level
             } else {
                 auto bx = [&](int c) { return c < 0 ? boxes[(uint32_t)keys[~c]] : ibox[c]; };
                 ibox[i] = box_union(bx(left[i]), bx(right[i]));
@@ -154,8 +173,9 @@ int emul_render(const rtg_scene_desc *d, const rtg_params *p, int threads, const
     const int total = tiles ? n_tiles : tx * ty;
     std::atomic<int> next{0};
     std::atomic<uint64_t> c0{0}, c1{0}, c2{0}, c3{0};
+    g_max_sp = 0;
     auto work = [&]() {
-        ArrayStack st;
+        TrackStack st;
         RayCounts rc{0, 0, 0, 0};
         for (;;) {
             const int k = next.fetch_add(1);
@@ -208,5 +228,8 @@ int emul_render(const rtg_scene_desc *d, const rtg_params *p, int threads, const
     if (counts) counts[0] = c0, counts[1] = c1, counts[2] = c2, counts[3] = c3;
     return 0;
 }
+
+// High-water mark of the traversal stack over the last emul_render.
+int emul_max_stack(void) { return g_max_sp.load(); }
 
 }  // extern "C"

@@ -69,6 +69,20 @@ def test_refraction_spp4_terrain_scene():
                                  o["stats"].rays_refract]
 
 
+def test_traversal_stack_stays_within_device_bound():
+    """Regression: an any-hit walk that stops early used to leave entries on the
+    per-thread stack, so later walks started deeper and the stack reached 64
+    entries (the device stack's capacity) on this scene.  Every walk now starts
+    empty and the stack never exceeds the tree depth."""
+    hs = rt.HostScene.workload(5, width=480, height=270, detail=8, n_spheres=64, n_lights=16)
+    p = rt.params(480, 270, max_level=8, spp=4, seed=5)
+    tiles = sample_tiles(480, 270, 4)
+    e = emul_render(hs.desc_ptr, p, tiles=tiles)
+    assert e["depth"] <= 63  # Karras tree over 62-bit unique keys
+    assert e["max_stack"] < e["depth"]
+    assert e["max_stack"] <= 32  # fits the LDS part of the device stack on this scene
+
+
 def test_tile_compat_leaves_partial_tiles():
     hs = rt.HostScene.workload(1, width=200, height=150)
     p = rt.params(200, 150, max_level=3, tile_compat=1)

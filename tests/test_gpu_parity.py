@@ -28,7 +28,10 @@ def _check(stats, min_ids=0.9999):
     assert stats["rgb_over_1"] == 0, stats
 
 
-@pytest.mark.parametrize("engine", [1])
+ENGINES = [1, 2]  # megakernel, wavefront
+
+
+@pytest.mark.parametrize("engine", ENGINES)
 def test_workload1_full_frame(rt_gpu, engine):
     rt = rt_gpu
     log(f"case: workload 1, 640x480 full frame, engine {engine}")
@@ -58,11 +61,12 @@ def test_workload1_tile_compat(rt_gpu):
     _check(s)
 
 
-def test_workload2_reduced_res_full_mesh(rt_gpu):
+@pytest.mark.parametrize("engine", ENGINES)
+def test_workload2_reduced_res_full_mesh(rt_gpu, engine):
     rt = rt_gpu
-    log("case: workload 2, 480x270, 200k-triangle icosphere, 64 spheres")
+    log(f"case: workload 2, 480x270, 200k-triangle icosphere, 64 spheres, engine {engine}")
     hs = rt.HostScene.workload(2, width=480, height=270)
-    p = rt.params(480, 270, max_level=4)
+    p = rt.params(480, 270, max_level=4, engine=engine)
     ds = rt.DeviceScene(hs)
     log(f"  bvh build {ds.build_ms:.2f} ms, scene bytes {ds.nbytes}")
     g = ds.render(p)
@@ -72,11 +76,12 @@ def test_workload2_reduced_res_full_mesh(rt_gpu):
     _check(s)
 
 
-def test_workload2_1080p_sampled_tiles(rt_gpu):
+@pytest.mark.parametrize("engine", ENGINES)
+def test_workload2_1080p_sampled_tiles(rt_gpu, engine):
     rt = rt_gpu
-    log("case: workload 2, 1920x1080, oracle on a 1/16 tile sample")
+    log(f"case: workload 2, 1920x1080, oracle on a 1/16 tile sample, engine {engine}")
     hs = rt.HostScene.workload(2)
-    p = rt.params(1920, 1080, max_level=4)
+    p = rt.params(1920, 1080, max_level=4, engine=engine)
     g = rt.DeviceScene(hs).render(p)
     tiles = sample_tiles(1920, 1080, 16)
     o = _oracle(hs, p, tiles=tiles)
@@ -137,3 +142,45 @@ def test_raytracer_drop_in(rt_gpu):
     assert d <= 1
     assert (buf[640 * 480 * 3:] == 127).all()
     assert use_time > 0
+
+
+@pytest.mark.parametrize("case", ["w2_1080p", "w5_refraction_spp4", "w1_tile_compat", "w3_heightfield"])
+def test_engines_bit_identical(rt_gpu, case):
+    """Megakernel and wavefront compose every ray-tree node with the same
+    operations: frames, ids and ray counts must be identical."""
+    rt = rt_gpu
+    log(f"case: megakernel vs wavefront, {case}")
+    if case == "w2_1080p":
+        hs, p = rt.HostScene.workload(2), dict(width=1920, height=1080, max_level=4)
+    elif case == "w5_refraction_spp4":
+        hs = rt.HostScene.workload(5, width=480, height=270, detail=8, n_spheres=64, n_lights=16)
+        p = dict(width=480, height=270, max_level=8, spp=4, seed=5)
+    elif case == "w1_tile_compat":
+        hs, p = rt.HostScene.workload(1, width=650, height=500), dict(width=650, height=500, max_level=3, tile_compat=1)
+    else:
+        hs = rt.HostScene.workload(3, index=1, width=320, height=256, detail=200000)
+        p = dict(width=320, height=256, max_level=hs.info.max_level)
+    ds = rt.DeviceScene(hs)
+    a = ds.render(rt.params(engine=1, **p))
+    b = ds.render(rt.params(engine=2, **p))
+    assert (a["rgb"] == b["rgb"]).all()
+    assert (a["ids"] == b["ids"]).all() and (a["t"] == b["t"]).all()
+    sa, sb = a["stats"], b["stats"]
+    assert (sa.rays_primary, sa.rays_shadow, sa.rays_reflect, sa.rays_refract) == (
+        sb.rays_primary, sb.rays_shadow, sb.rays_reflect, sb.rays_refract)
+    log(f"  rays {sa.rays_total}  mega {sa.render_ms:.3f} ms  wave {sb.render_ms:.3f} ms")
+
+
+def test_workload5_reduced_vs_oracle(rt_gpu):
+    """Stress features (refraction through sphere meshes, 16 lights, 4 jittered
+    samples, depth 8) at reduced size against the oracle on a tile sample."""
+    rt = rt_gpu
+    log("case: workload 5 reduced (960x540, terrain /4, 256 sphere meshes), oracle on 1/8 tiles")
+    hs = rt.HostScene.workload(5, width=960, height=540, detail=4)
+    p = rt.params(960, 540, max_level=8, spp=4, seed=5, engine=2)
+    g = rt.DeviceScene(hs).render(p)
+    tiles = sample_tiles(960, 540, 8)
+    o = _oracle(hs, p, tiles=tiles)
+    s = compare_frames(g, o, region=tile_mask(960, 540, tiles))
+    log(f"  {s}  rays {g['stats'].rays_total}")
+    _check(s)
