@@ -111,7 +111,9 @@ static int render_device(SceneRes &S, const rtg_params &p, int W, int H, const R
     k_zero_u64<<<1, 64, 0, stream>>>(o.counts, 4);
     if (p.mode != RTG_MODE_RAYTRACE && p.mode != RTG_MODE_REFRACTTEST && p.mode != 0)
         return fail(RTG_E_INVALID, "render mode not supported by this build");
-    int engine = p.engine == RTG_ENGINE_AUTO ? RTG_ENGINE_MEGAKERNEL : p.engine;
+    // AUTO: the wavefront engine (faster on every measured workload; the
+    // megakernel's per-thread ray trees diverge on deep refraction trees).
+    int engine = p.engine == RTG_ENGINE_AUTO ? RTG_ENGINE_WAVEFRONT : p.engine;
     if (engine == RTG_ENGINE_WAVEFRONT) return launch_wavefront(S, p, W, H, o, stream, A);
     return launch_megakernel(S, p, W, H, o, stream, A);
 }
