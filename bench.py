@@ -44,7 +44,18 @@ def main() -> int:
     ap.add_argument("--engine", default="mega", choices=["mega", "wave", "auto"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-tiles", type=int, default=32, help="1/N of the 64x64 tiles for the CPU baseline sample")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (reference for FETCH_SIZE / WRITE_SIZE scaling)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        a = torch.empty(1 << 30, dtype=torch.uint8, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "copy", "bytes": 1 << 30, "calls": 3}))
+        return 0
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(frames and requests shard across GPUs without exchange; see the design notes)", file=sys.stderr)
@@ -102,9 +113,10 @@ def main() -> int:
     osc = po.OracleScene(hs.desc_ptr)
     tx, ty = (W + 63) // 64, (H + 63) // 64
     rng = np.random.default_rng(11)
-    tiles = np.sort(rng.choice(tx * ty, size=max(1, (tx * ty) // args.cpu_tiles), replace=False)).astype(np.int32)
     threads = max(1, min(64, os.cpu_count() or 1))
-    tr = osc.traffic(p, threads=threads, tiles=tiles)
+    traffic_tiles = np.sort(rng.choice(tx * ty, size=(tx * ty) // 16, replace=False)).astype(np.int32)
+    tr = osc.traffic(p, threads=threads, tiles=traffic_tiles)
+    tiles = traffic_tiles[:: max(1, args.cpu_tiles // 16)]
     bpr = algorithmic_bytes_per_ray(tr)
     achieved = rays_per_frame * bpr / (kernel_ms / 1000.0) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -74,9 +74,12 @@ int rtg_host_set_env_light(rtg_host_scene_t scene, const float env[4]);
 int rtg_host_resize(rtg_host_scene_t scene, int width, int height);
 
 /* RayTracer::start(type, tnum) followed by waiting for isFinish
- * (RayTracer.cpp:614-696): renders into `out` (2048*2048*3 bytes, the
- * reference's fixed buffer) and reports useTime in seconds. */
-int rtg_host_raytrace(rtg_host_scene_t scene, int type, int max_level, uint8_t *out, double *use_time);
+ * (RayTracer.cpp:614-696), `frames` times on one RayTracer, as the viewer
+ * re-renders (main.cpp:323-337): the last frame lands in `out` (2048*2048*3
+ * bytes, the reference's fixed buffer); useTime of the last frame in seconds;
+ * `uploads` (optional) receives how many times the scene was uploaded. */
+int rtg_host_raytrace(rtg_host_scene_t scene, int type, int max_level, int frames, uint8_t *out,
+                      double *use_time, uint64_t *uploads);
 
 #ifdef __cplusplus
 }

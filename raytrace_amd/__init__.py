@@ -123,13 +123,15 @@ class HostScene:
     def resize(self, w: int, h: int) -> None:
         check(self._lib.rtg_host_resize(self.h, w, h), "Camera::resize")
 
-    def raytrace(self, type_: int = RTG_MODE_RAYTRACE, max_level: int = 1):
-        """RayTracer::start + wait for isFinish; returns (2048x2048x3 buffer, useTime)."""
+    def raytrace(self, type_: int = RTG_MODE_RAYTRACE, max_level: int = 1, frames: int = 1, with_uploads=False):
+        """RayTracer::start + wait for isFinish, `frames` times on one RayTracer;
+        returns (2048x2048x3 buffer of the last frame, its useTime[, scene uploads])."""
         out = np.full((2048 * 2048 * 3,), 127, np.uint8)
         t = C.c_double()
-        check(self._lib.rtg_host_raytrace(self.h, type_, max_level, out.ctypes.data_as(C.c_void_p), C.byref(t)),
-              "RayTracer::start")
-        return out, t.value
+        up = C.c_uint64()
+        check(self._lib.rtg_host_raytrace(self.h, type_, max_level, frames, out.ctypes.data_as(C.c_void_p),
+                                          C.byref(t), C.byref(up)), "RayTracer::start")
+        return (out, t.value, up.value) if with_uploads else (out, t.value)
 
 
 class DeviceScene:

@@ -175,7 +175,7 @@ SIGNATURES = {
     "rtg_host_set_light_position": (_I, [_P, _I, C.POINTER(_F)]),
     "rtg_host_set_env_light": (_I, [_P, C.POINTER(_F)]),
     "rtg_host_resize": (_I, [_P, _I, _I]),
-    "rtg_host_raytrace": (_I, [_P, _I, _I, _P, C.POINTER(C.c_double)]),
+    "rtg_host_raytrace": (_I, [_P, _I, _I, _I, _P, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
 }
 
 _lib = None

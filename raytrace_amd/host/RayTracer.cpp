@@ -42,8 +42,15 @@ void RayTracer::start(const uint8_t type, const int8_t tnum) {
     int st = rtg_init(0);
     if (st != RTG_OK) return fail(st);
     SceneFlattener::flatten(*scene, *flat);
-    st = devScene == nullptr ? rtg_scene_create(&flat->desc, &devScene) : rtg_scene_update(devScene, &flat->desc);
-    if (st != RTG_OK) return fail(st);
+    // The scene stays resident between frames; it is re-uploaded (and its BVH
+    // rebuilt) only when the flattened content changed.
+    const uint64_t fp = rtg_desc_fingerprint(flat->desc);
+    if (devScene == nullptr || fp != devFingerprint) {
+        st = devScene == nullptr ? rtg_scene_create(&flat->desc, &devScene) : rtg_scene_update(devScene, &flat->desc);
+        if (st != RTG_OK) return fail(st);
+        devFingerprint = fp;
+        ++uploads;
+    }
     rtg_params p{};
     p.mode = type;
     p.max_level = maxLevel;

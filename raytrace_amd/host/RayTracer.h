@@ -44,6 +44,7 @@ public:
     int engine = RTG_ENGINE_AUTO;
     volatile int lastStatus = RTG_OK;
     rtg_stats lastStats{};
+    uint64_t uploads = 0;  // scene uploads to the device (one per change of the scene)
 
     RayTracer(Scene &scene);
     ~RayTracer();

@@ -131,14 +131,19 @@ int rtg_host_resize(rtg_host_scene_t h, int w, int hh) {
     return RTG_OK;
 }
 
-int rtg_host_raytrace(rtg_host_scene_t h, int type, int max_level, uint8_t *out, double *use_time) {
-    if (!valid(h) || out == nullptr) return RTG_E_INVALID;
+int rtg_host_raytrace(rtg_host_scene_t h, int type, int max_level, int frames, uint8_t *out, double *use_time,
+                      uint64_t *uploads) {
+    if (!valid(h) || out == nullptr || frames < 1) return RTG_E_INVALID;
     RayTracer rt(*h->scene);
     rt.maxLevel = (uint8_t)max_level;
-    rt.start((uint8_t)type, 4);
-    while (!rt.isFinish) std::this_thread::sleep_for(std::chrono::milliseconds(1));
+    for (int f = 0; f < frames; ++f) {
+        rt.start((uint8_t)type, 4);
+        while (!rt.isFinish) std::this_thread::sleep_for(std::chrono::milliseconds(1));
+        if (rt.lastStatus != RTG_OK) break;
+    }
     memcpy(out, rt.output, 2048 * 2048 * 3);
     if (use_time != nullptr) *use_time = rt.useTime;
+    if (uploads != nullptr) *uploads = rt.uploads;
     return rt.lastStatus;
 }
 

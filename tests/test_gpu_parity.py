@@ -142,6 +142,10 @@ def test_raytracer_drop_in(rt_gpu):
     assert d <= 1
     assert (buf[640 * 480 * 3:] == 127).all()
     assert use_time > 0
+    # re-rendering an unchanged scene keeps it resident: one upload for three frames
+    buf3, _, uploads = hs.raytrace(0x80, 3, frames=3, with_uploads=True)
+    assert uploads == 1
+    assert (buf3 == buf).all()
 
 
 @pytest.mark.parametrize("case", ["w2_1080p", "w5_refraction_spp4", "w1_tile_compat", "w3_heightfield"])
