@@ -6,7 +6,7 @@
 namespace rtg {
 
 constexpr int kBlockThreads = 128;  // two wave64s
-constexpr int kLdsStack = 32;       // entries per thread in LDS
+constexpr int kLdsStack = 16;       // entries per thread in LDS (8 KB per block)
 constexpr int kStackMax = 64;       // a Karras tree over 62-bit unique keys is < 63 levels deep
 
 // Per-thread stack: the first kLdsStack entries in LDS (thread-interleaved, no
@@ -53,6 +53,39 @@ __device__ inline ChunkRange xcd_chunks(int nchunks) {
     return ChunkRange{start + local, min(nchunks, start + region), per};
 }
 
+// Device counter alone in its 256-byte line: counters that many waves hit
+// with atomics must not share a line (same-line atomics serialise).
+struct alignas(256) LineCtr {
+    unsigned v;
+    unsigned pad[63];
+};
+
+// Per-wave dynamic work split: a wave takes chunks of 64 items, first from its
+// XCD's contiguous eighth of the range (one counter per XCD, so neighbouring
+// chunks share an L2), then from the other XCDs' eighths once its own is
+// exhausted.  `ctr` points to 8 counters that are zero at launch.  Every lane
+// of the wave must call next(); it returns the chunk's first item, or -1
+// when every chunk is taken (each wave stops after at most region + 8 calls).
+struct WaveQueue {
+    LineCtr *ctr;
+    int nchunks, region, x, left;
+    __device__ WaveQueue(LineCtr *c, int n_items)
+        : ctr(c), nchunks((n_items + 63) >> 6), region((((n_items + 63) >> 6) + 7) >> 3), x(blockIdx.x & 7),
+          left(8) {}
+    __device__ int next() {
+        while (left > 0) {
+            int c = 0;
+            if (__lane_id() == 0) c = (int)atomicAdd(&ctr[x].v, 1u);
+            c = __shfl(c, 0, 64);
+            const int chunk = x * region + c;
+            if (c < region && chunk < nchunks) return chunk << 6;
+            x = (x + 1) & 7;
+            --left;
+        }
+        return -1;
+    }
+};
+
 __device__ inline uint32_t wave_sum_u32(uint32_t v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
@@ -72,6 +105,16 @@ __device__ inline int wave_append(unsigned *counter, int count) {
     if (lane == 63 && total > 0) base = (int)atomicAdd(counter, (unsigned)total);
     base = __shfl(base, 63, 64);
     return base + x - count;
+}
+
+// Wave-aggregated append of 0 or 1 item per lane (ballot + bit count).
+__device__ inline int wave_append1(unsigned *counter, bool take) {
+    const unsigned long long m = __ballot(take);
+    const int lane = __lane_id();
+    int base = 0;
+    if (lane == 0 && m != 0ull) base = (int)atomicAdd(counter, (unsigned)__popcll(m));
+    base = __shfl(base, 0, 64);
+    return base + __popcll(m & ((1ull << lane) - 1ull));
 }
 
 }  // namespace rtg

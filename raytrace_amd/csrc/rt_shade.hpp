@@ -137,9 +137,27 @@ RTG_HD LightTerm light_term(const DevLight &L, V3 P) {
     return lt;
 }
 
+// Diffuse and specular factors of a light at a hit (RayTracer.cpp:525-540):
+// n_l = N.l, n_h = N.h with h the half vector of l and -D.  A light adds
+// nothing unless one of them is positive, whether or not it is blocked, so
+// its shadow ray need not be traced (it is still counted: the reference
+// traces it).
+struct LightDots {
+    float n_l, n_h;
+    RTG_HD bool matters() const { return n_l > 0 || n_h > 0; }
+};
+RTG_HD LightDots light_dots(V3 N, V3 D, const LightTerm &lt) {
+    LightDots d;
+    d.n_l = dot(N, lt.p2l);
+    d.n_h = dot(N, normalize(lt.p2l - D));
+    return d;
+}
+
 // Local colour vc (kd...+ka...) + specular (RayTracer.cpp:469-546) at hit
-// point P with normal N, texel colour vc, incoming direction D.  `blocked`
-// answers the shadow test of light li for the given shadow ray.
+// point P with normal N, texel colour vc, incoming direction D.
+// `blocked(li, lt, matters)` answers the shadow test of light li; it is
+// called for every switched-on light (so it can count the ray), and its
+// answer is used only when `matters`.
 template <class Blocked>
 RTG_HD V3 shade_local(const SceneView &S, V3 D, V3 P, V3 N, int mtl, V3 vc, Blocked &&blocked) {
     const DevMaterial &m = S.mtls[mtl];
@@ -150,12 +168,10 @@ RTG_HD V3 shade_local(const SceneView &S, V3 D, V3 P, V3 N, int mtl, V3 vc, Bloc
         if (!L.on) continue;
         const LightTerm lt = light_term(L, P);
         va = va + mixmul(m.ka, lt.la);
-        if (blocked(li, lt)) continue;
-        float n_n = dot(N, lt.p2l);
-        if (n_n > 0) vd = vd + mixmul(m.kd, lt.ld) * n_n;
-        const V3 h = normalize(lt.p2l - D);
-        n_n = dot(N, h);
-        if (n_n > 0) vs = vs + mixmul(m.ks, lt.ls) * powf(n_n, m.shiness);
+        const LightDots ld = light_dots(N, D, lt);
+        if (blocked(li, lt, ld.matters()) || !ld.matters()) continue;
+        if (ld.n_l > 0) vd = vd + mixmul(m.kd, lt.ld) * ld.n_l;
+        if (ld.n_h > 0) vs = vs + mixmul(m.ks, lt.ls) * powf(ld.n_h, m.shiness);
     }
     return mixmul(vc, vd + va) + vs;
 }
@@ -279,8 +295,9 @@ RTG_HD V3 render_sample(const SceneView &S, int max_level, const TraceRay &prima
                 ret = V3{0, 0, 0}, ret_a = kMiss;
             } else {
                 const HitRec hr = make_hitrec(S, q, h);
-                const V3 L = local_color(S, q.r.d, hr, [&](int, const LightTerm &lt) -> bool {
+                const V3 L = local_color(S, q.r.d, hr, [&](int, const LightTerm &lt, bool matters) -> bool {
                     rc.shadow++;
+                    if (!matters) return false;
                     TraceRay sr;
                     sr.r.o = hr.P;
                     sr.r.d = lt.p2l;

@@ -35,6 +35,15 @@ struct WNode {  // 64 B
     int bits;  // bits 0-23 object, 24-26 copy bin, 27 valid, 28 inside, 29 total internal reflection
 };
 
+// Device counters of one level, zeroed at the start of a frame; each in its
+// own 256-byte line.
+struct LevelCtr {
+    LineCtr n_next;       // child rays appended for level k + 1
+    LineCtr n_shadow;     // shadow-ray entries appended
+    LineCtr sched[3][8];  // WaveQueue counters of extend, shade, occlude
+};
+constexpr int kCtrWords = (int)(sizeof(LevelCtr) / sizeof(unsigned));
+
 struct WaveArgs {
     int W, H, spp, max_level, tile_compat;
     int bw;         // 8x8 pixel blocks per row
@@ -115,14 +124,14 @@ __device__ inline void add_counts(unsigned long long *counts, uint32_t a, uint32
 }
 
 __global__ __launch_bounds__(kBlockThreads) void k_extend(SceneView S, WaveArgs A, int k, const WRay *rays,
-                                                          const unsigned *n_ptr, WHit *hits) {
+                                                          const unsigned *n_ptr, WHit *hits, LineCtr *sched) {
     __shared__ int lds[kLdsStack * kBlockThreads];
     DevStack st = make_stack(lds, A.spill);
     const int n = k == 0 ? A.n_samples : (int)*n_ptr;
-    const ChunkRange cr = xcd_chunks((n + kBlockThreads - 1) / kBlockThreads);
+    WaveQueue wq(sched, n);
     uint32_t c_prim = 0, c_refl = 0, c_refr = 0;
-    for (int c = cr.first; c < cr.end; c += cr.step) {
-        const int i = c * kBlockThreads + threadIdx.x;
+    for (int base = wq.next(); base >= 0; base = wq.next()) {
+        const int i = base + __lane_id();
         if (i >= n) continue;
         TraceRay q;
         int px = 0, py = 0, si = 0;
@@ -160,16 +169,18 @@ __global__ __launch_bounds__(kBlockThreads) void k_extend(SceneView S, WaveArgs 
 
 __global__ __launch_bounds__(kBlockThreads) void k_shade(SceneView S, WaveArgs A, int k, const WRay *rays,
                                                          const unsigned *n_ptr, const WHit *hits, WNode *nodes,
-                                                         unsigned *masks, WRay *next, unsigned *n_next,
-                                                         unsigned *shadow, unsigned *n_shadow, uint32_t lights_on) {
+                                                         unsigned *masks, WRay *next, LevelCtr *ctr, unsigned *shadow,
+                                                         uint32_t lights_on) {
     const int n = k == 0 ? A.n_samples : (int)*n_ptr;
     const int n_on = __builtin_popcount(lights_on);
-    const ChunkRange cr = xcd_chunks((n + kBlockThreads - 1) / kBlockThreads);
+    WaveQueue wq(ctr->sched[1], n);
     uint32_t c_shadow = 0;
-    for (int c = cr.first; c < cr.end; c += cr.step) {
-        const int i = c * kBlockThreads + threadIdx.x;
-        int n_child = 0, n_sh = 0;
-        WRay child[2];
+    for (int base = wq.next(); base >= 0; base = wq.next()) {
+        const int i = base + __lane_id();
+        int n_child = 0;
+        bool lit = false;
+        WRay c0, c1;  // children in append order (registers, no indexed array)
+        V3 P = V3{0, 0, 0}, N = V3{0, 0, 0}, D = V3{0, 0, 0};
         if (i < n) {
             TraceRay q;
             float bwc = 1.0f, mtlrfr = 1.0f;
@@ -191,14 +202,22 @@ __global__ __launch_bounds__(kBlockThreads) void k_shade(SceneView S, WaveArgs A
                 const V3 vc = texel(S, hr.tex, hr.tu, hr.tv);
                 const DevMaterial &m = S.mtls[hr.mtl];
                 int tir = 0;
-                if (m.reflect > 0.01f && k + 1 <= A.max_level && bwc * m.reflect >= 1e-5f)
-                    child[n_child++] = make_wray(reflect_ray(q, hr), bwc * m.reflect, 1.0f, 2 * i);
+                if (m.reflect > 0.01f && k + 1 <= A.max_level && bwc * m.reflect >= 1e-5f) {
+                    c0 = make_wray(reflect_ray(q, hr), bwc * m.reflect, 1.0f, 2 * i);
+                    n_child = 1;
+                }
                 if (m.refract > 0.01f) {
                     TraceRay cq;
-                    if (!refract_ray(q, mtlrfr, hr, cq))
+                    if (!refract_ray(q, mtlrfr, hr, cq)) {
                         tir = 1;
-                    else if (k + 1 <= A.max_level && bwc * m.refract >= 1e-5f)
-                        child[n_child++] = make_wray(cq, bwc * m.refract, hr.rfr, 2 * i + 1);
+                    } else if (k + 1 <= A.max_level && bwc * m.refract >= 1e-5f) {
+                        const WRay w = make_wray(cq, bwc * m.refract, hr.rfr, 2 * i + 1);
+                        if (n_child == 0)
+                            c0 = w;
+                        else
+                            c1 = w;
+                        ++n_child;
+                    }
                 }
                 nd.px = hr.P.x, nd.py = hr.P.y, nd.pz = hr.P.z;
                 nd.nx = hr.N.x, nd.ny = hr.N.y, nd.nz = hr.N.z;
@@ -209,19 +228,38 @@ __global__ __launch_bounds__(kBlockThreads) void k_shade(SceneView S, WaveArgs A
                 nd.id_prim = hr.id.prim;
                 nd.bits = (hr.id.obj & 0xFFFFFF) | ((hr.id.bin & 7) << 24) | (1 << 27) | ((hr.inside & 1) << 28) |
                           (tir << 29);
-                n_sh = n_on;
+                lit = true;
+                P = hr.P, N = hr.N, D = q.r.d;
+                c_shadow += n_on;  // the reference traces one shadow ray per switched-on light
             }
             nodes[i] = nd;
             masks[i] = 0;
         }
-        const int cb = wave_append(n_next, n_child);
-        for (int j = 0; j < n_child; ++j) next[cb + j] = child[j];
-        const int sb = wave_append(n_shadow, n_sh);
-        if (n_sh) {
-            int j = 0;
-            for (uint32_t m = lights_on; m; m &= m - 1) shadow[sb + j++] = ((unsigned)i << 5) | (unsigned)__builtin_ctz(m);
+        const int cb = wave_append(&ctr->n_next.v, n_child);
+        if (n_child > 0) next[cb] = c0;
+        if (n_child > 1) next[cb + 1] = c1;
+        // Shadow entries (node << 5 | light), light-major within the wave's
+        // block, so that a wave of k_occlude traces neighbouring points toward
+        // one light.  Lights that cannot change the colour are not traced
+        // (shade_local ignores their mask bit).
+        uint32_t emit = 0;
+        if (lit) {
+            int r = 0;
+            for (uint32_t lm = lights_on; lm; lm &= lm - 1, ++r)
+                if (light_dots(N, D, light_term(S.lights[__builtin_ctz(lm)], P)).matters()) emit |= 1u << r;
         }
-        c_shadow += n_sh;
+        int total = 0;
+        for (int r = 0; r < n_on; ++r) total += __popcll(__ballot((emit >> r) & 1u));
+        int sb = 0;
+        if (__lane_id() == 0 && total > 0) sb = (int)atomicAdd(&ctr->n_shadow.v, (unsigned)total);
+        sb = __shfl(sb, 0, 64);
+        const unsigned long long below = (1ull << __lane_id()) - 1ull;
+        int r = 0;
+        for (uint32_t lm = lights_on; lm; lm &= lm - 1, ++r) {
+            const unsigned long long m = __ballot((emit >> r) & 1u);
+            if ((emit >> r) & 1u) shadow[sb + __popcll(m & below)] = ((unsigned)i << 5) | (unsigned)__builtin_ctz(lm);
+            sb += __popcll(m);
+        }
     }
     add_counts(A.counts, 0, c_shadow, 0, 0);
 }
@@ -229,14 +267,13 @@ __global__ __launch_bounds__(kBlockThreads) void k_shade(SceneView S, WaveArgs A
 __device__ inline SrcId node_id(const WNode &nd) { return SrcId{nd.bits & 0xFFFFFF, nd.id_prim, (nd.bits >> 24) & 7}; }
 
 __global__ __launch_bounds__(kBlockThreads) void k_occlude(SceneView S, WaveArgs A, const WNode *nodes,
-                                                           const unsigned *shadow, const unsigned *n_shadow,
-                                                           unsigned *masks) {
+                                                           const unsigned *shadow, LevelCtr *ctr, unsigned *masks) {
     __shared__ int lds[kLdsStack * kBlockThreads];
     DevStack st = make_stack(lds, A.spill);
-    const int n = (int)*n_shadow;
-    const ChunkRange cr = xcd_chunks((n + kBlockThreads - 1) / kBlockThreads);
-    for (int c = cr.first; c < cr.end; c += cr.step) {
-        const int j = c * kBlockThreads + threadIdx.x;
+    const int n = (int)ctr->n_shadow.v;
+    WaveQueue wq(ctr->sched[2], n);
+    for (int base = wq.next(); base >= 0; base = wq.next()) {
+        const int j = base + __lane_id();
         if (j >= n) continue;
         const unsigned e = shadow[j];
         const int i = (int)(e >> 5), li = (int)(e & 31);
@@ -275,7 +312,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_combine(SceneView S, WaveArgs
             const unsigned mask = masks[i];
             const V3 L = shade_local(S, V3{nd.dx, nd.dy, nd.dz}, V3{nd.px, nd.py, nd.pz}, V3{nd.nx, nd.ny, nd.nz},
                                      nd.mtl, V3{nd.vr, nd.vg, nd.vb},
-                                     [&](int li, const LightTerm &) -> bool { return (mask >> li) & 1u; });
+                                     [&](int li, const LightTerm &, bool) -> bool { return (mask >> li) & 1u; });
             const DevMaterial &m = S.mtls[nd.mtl];
             const float4 a = slots[2 * i], b = slots[2 * i + 1];
             const V3 col = compose(L, m.reflect, m.refract, (nd.bits >> 29) & 1, (nd.bits >> 28) & 1, m.kd,
@@ -309,7 +346,17 @@ __global__ void k_resolve(WaveArgs A, const float4 *samples, uint8_t *rgb) {
 }
 
 __global__ void k_zero_u32(unsigned *p, int n) {
-    if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// Resident blocks per CU of a kernel at kBlockThreads threads (registers and
+// LDS permitting), queried once.
+template <class K>
+static int blocks_per_cu(K kernel) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, kBlockThreads, 0) != hipSuccess || nb < 1) nb = 4;
+    return nb;
 }
 
 int launch_wavefront(const SceneRes &S, const rtg_params &p, int W, int H, const RenderOut &o, hipStream_t stream,
@@ -317,12 +364,15 @@ int launch_wavefront(const SceneRes &S, const rtg_params &p, int W, int H, const
     double dp;
     int st = upload_raygen_tables(S, W, H, stream, A, dp);
     if (st != RTG_OK) return st;
-    const int blocks = persistent_blocks(A);
-    if (ensure(A.spill, spill_ints(blocks) * sizeof(int))) return fail(RTG_E_NOMEM, "spill");
+    const int cus = A.cus > 0 ? A.cus : 256;
+    static const int occ_extend = blocks_per_cu(k_extend), occ_shade = blocks_per_cu(k_shade),
+                     occ_occlude = blocks_per_cu(k_occlude);
+    const int g_extend = cus * occ_extend, g_shade = cus * occ_shade, g_occlude = cus * occ_occlude;
+    if (ensure(A.spill, spill_ints(std::max(g_extend, g_occlude)) * sizeof(int))) return fail(RTG_E_NOMEM, "spill");
+    constexpr size_t kCtrBytes = (size_t)Arena::kLevels * sizeof(LevelCtr);
     if (A.qcount_host == nullptr)
-        RTG_HIP(hipHostMalloc(reinterpret_cast<void **>(&A.qcount_host), 2 * Arena::kLevels * sizeof(unsigned),
-                              hipHostMallocDefault));
-    if (ensure(A.w_qcount, 2 * Arena::kLevels * sizeof(unsigned))) return fail(RTG_E_NOMEM, "queue counters");
+        RTG_HIP(hipHostMalloc(reinterpret_cast<void **>(&A.qcount_host), kCtrBytes, hipHostMallocDefault));
+    if (ensure(A.w_qcount, kCtrBytes)) return fail(RTG_E_NOMEM, "queue counters");
     WaveArgs a;
     a.W = W, a.H = H, a.spp = p.spp > 0 ? p.spp : 1, a.max_level = p.max_level, a.tile_compat = p.tile_compat;
     a.bw = (W + 7) / 8;
@@ -336,8 +386,10 @@ int launch_wavefront(const SceneRes &S, const rtg_params &p, int W, int H, const
     a.spill = A.spill.as<int>();
     const uint32_t lights_on = S.lights_on;  // lights that trace shadow rays (Light::bLight)
     const int n_on = __builtin_popcount(lights_on);
-    unsigned *qc = A.w_qcount.as<unsigned>();
-    k_zero_u32<<<1, 64, 0, stream>>>(qc, 2 * Arena::kLevels);
+    LevelCtr *ctr = A.w_qcount.as<LevelCtr>();
+    const int ctr_words = Arena::kLevels * kCtrWords;
+    k_zero_u32<<<(ctr_words + 255) / 256, 256, 0, stream>>>(A.w_qcount.as<unsigned>(), ctr_words);
+    auto grid = [](int g, size_t items) { return (int)std::max<size_t>(8, std::min<size_t>(g, (items + 127) / 128)); };
     size_t n_k = (size_t)a.n_samples;
     int last = -1;
     for (int k = 0; k <= p.max_level && n_k > 0; ++k) {
@@ -349,30 +401,31 @@ int launch_wavefront(const SceneRes &S, const rtg_params &p, int W, int H, const
             ensure(A.w_rays[k + 1 < Arena::kLevels ? k + 1 : k], cap_next * sizeof(WRay)))
             return fail(RTG_E_NOMEM, "wavefront queues");
         const WRay *rays = k == 0 ? nullptr : A.w_rays[k].as<WRay>();
-        const unsigned *n_ptr = k == 0 ? nullptr : qc + 2 * (k - 1);
+        const unsigned *n_ptr = k == 0 ? nullptr : &ctr[k - 1].n_next.v;
         WRay *next = A.w_rays[k + 1 < Arena::kLevels ? k + 1 : k].as<WRay>();
-        k_extend<<<blocks, kBlockThreads, 0, stream>>>(S.view, a, k, rays, n_ptr, A.w_hits[k].as<WHit>());
-        k_shade<<<blocks, kBlockThreads, 0, stream>>>(S.view, a, k, rays, n_ptr, A.w_hits[k].as<WHit>(),
-                                                      A.w_nodes[k].as<WNode>(), A.w_masks[k].as<unsigned>(), next,
-                                                      qc + 2 * k, A.w_shadow[k].as<unsigned>(), qc + 2 * k + 1,
-                                                      lights_on);
+        k_extend<<<grid(g_extend, n_k), kBlockThreads, 0, stream>>>(S.view, a, k, rays, n_ptr,
+                                                                    A.w_hits[k].as<WHit>(), ctr[k].sched[0]);
+        k_shade<<<grid(g_shade, n_k), kBlockThreads, 0, stream>>>(
+            S.view, a, k, rays, n_ptr, A.w_hits[k].as<WHit>(), A.w_nodes[k].as<WNode>(), A.w_masks[k].as<unsigned>(),
+            next, ctr + k, A.w_shadow[k].as<unsigned>(), lights_on);
         k_fill_slots<<<256, 256, 0, stream>>>(A.w_slots[k].as<float4>(), n_ptr, k == 0 ? a.n_samples : -1);
-        k_occlude<<<blocks, kBlockThreads, 0, stream>>>(S.view, a, A.w_nodes[k].as<WNode>(),
-                                                        A.w_shadow[k].as<unsigned>(), qc + 2 * k + 1,
-                                                        A.w_masks[k].as<unsigned>());
+        if (n_on > 0)
+            k_occlude<<<grid(g_occlude, n_k * n_on), kBlockThreads, 0, stream>>>(
+                S.view, a, A.w_nodes[k].as<WNode>(), A.w_shadow[k].as<unsigned>(), ctr + k,
+                A.w_masks[k].as<unsigned>());
         RTG_HIP(hipGetLastError());
         last = k;
         if (k == p.max_level) break;
-        RTG_HIP(hipMemcpyAsync(A.qcount_host, qc, 2 * Arena::kLevels * sizeof(unsigned), hipMemcpyDeviceToHost,
-                               stream));
+        RTG_HIP(hipMemcpyAsync(A.qcount_host, &ctr[k].n_next.v, sizeof(unsigned), hipMemcpyDeviceToHost, stream));
         RTG_HIP(hipStreamSynchronize(stream));
-        n_k = A.qcount_host[2 * k];
+        n_k = A.qcount_host[0];
     }
     if (*A.cancel_host) return RTG_OK;
     if (ensure(A.w_samples, (size_t)a.n_samples * sizeof(float4))) return fail(RTG_E_NOMEM, "samples");
+    const int blocks = std::max(8, cus * 8);
     for (int k = last; k >= 0; --k) {
         const WRay *rays = k == 0 ? nullptr : A.w_rays[k].as<WRay>();
-        const unsigned *n_ptr = k == 0 ? nullptr : qc + 2 * (k - 1);
+        const unsigned *n_ptr = k == 0 ? nullptr : &ctr[k - 1].n_next.v;
         k_combine<<<blocks, kBlockThreads, 0, stream>>>(
             S.view, a, k, rays, n_ptr, A.w_nodes[k].as<WNode>(), A.w_masks[k].as<unsigned>(),
             A.w_slots[k].as<float4>(), k > 0 ? A.w_slots[k - 1].as<float4>() : nullptr, A.w_samples.as<float4>());
