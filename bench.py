@@ -11,7 +11,7 @@ the reference recursion traced on the GPU, outputs written to device memory.
 Ray counts are the reference-semantic counts accumulated by the kernels
 (tested equal to the oracle's).  Prints one JSON line.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--engine mega|wave]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--engine auto|mega|wave]
 """
 from __future__ import annotations
 
@@ -28,12 +28,38 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
-def algorithmic_bytes_per_ray(tr) -> float:
-    """32 B ray read + 32 B hit/payload write + 64 B per node visit + 48 B per
-    triangle test + 16 B per sphere test, with the visit/test counts of the
-    canonical oracle BVH on the frame's ray set."""
-    rays = max(1, tr.rays)
-    return 32 + 32 + 64.0 * tr.node_visits / rays + 48.0 * tr.tri_tests / rays + 16.0 * tr.sph_tests / rays
+def ray_bytes(rays, node_visits, tri_tests, sph_tests) -> float:
+    """Algorithmic bytes of `rays` traced rays: 32 B ray read + 32 B hit/payload
+    write per ray + 64 B per node visit + 48 B per triangle test + 16 B per
+    sphere test, with the visit/test counts of the canonical oracle BVH."""
+    return 64.0 * rays + 64.0 * node_visits + 48.0 * tri_tests + 16.0 * sph_tests
+
+
+def traffic_model(tr) -> dict:
+    """Per-ray algorithmic bytes by ray kind from an oracle traffic count (shadow
+    rays whose outcome cannot change the colour are not traced and cost 0)."""
+    ref_rays = tr.rays + tr.shadow_skipped  # reference-semantic rays of the sample
+    shadow = ray_bytes(tr.shadow_rays, tr.shadow_node_visits, tr.shadow_tri_tests, tr.shadow_sph_tests)
+    closest = ray_bytes(tr.rays - tr.shadow_rays, tr.node_visits - tr.shadow_node_visits,
+                        tr.tri_tests - tr.shadow_tri_tests, tr.sph_tests - tr.shadow_sph_tests)
+    return {
+        "per_reference_ray": (shadow + closest) / max(1, ref_rays),
+        "per_shadow_ray": shadow / max(1, tr.shadow_rays),
+        "per_closest_ray": closest / max(1, tr.rays - tr.shadow_rays),
+        "shadow_skipped_frac": tr.shadow_skipped / max(1, tr.shadow_rays + tr.shadow_skipped),
+    }
+
+
+def pmc_traffic(engine: str, kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary of this workload (FETCH_SIZE and WRITE_SIZE passes, scaled by a
+    plain-copy calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_workload2_1080p.json")
+    try:
+        with open(path) as f:
+            return json.load(f)[engine][kernel]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def main() -> int:
@@ -41,7 +67,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--engine", default="mega", choices=["mega", "wave", "auto"])
+    ap.add_argument("--engine", default="auto", choices=["mega", "wave", "auto"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-tiles", type=int, default=32, help="1/N of the 64x64 tiles for the CPU baseline sample")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -68,6 +94,7 @@ def main() -> int:
     from raytrace_amd import _capi
 
     engine = {"mega": rt.RTG_ENGINE_MEGAKERNEL, "wave": rt.RTG_ENGINE_WAVEFRONT, "auto": rt.RTG_ENGINE_AUTO}[args.engine]
+    eng_name = "mega" if args.engine == "mega" else "wave"  # AUTO selects the wavefront engine
     rt.init(0)
     lib = _capi.load()
     W, H, DEPTH = 1920, 1080, 4
@@ -104,9 +131,23 @@ def main() -> int:
     stream.synchronize()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    kernel_ms = e0.elapsed_time(e1) / args.steps
+    frame_ms = e0.elapsed_time(e1) / args.steps
     ms_per_step = wall * 1000.0 / args.steps
     value = rays_per_frame / (ms_per_step / 1000.0) / 1e6
+
+    # Per-kernel device time: the same K frames again, with HIP events around
+    # every launch on the render stream (RTG_FLAG_KERNEL_TIMES).
+    pk = rt.params(W, H, max_level=DEPTH, engine=engine, flags=rt.RTG_FLAG_KERNEL_TIMES)
+    rt.kernel_times()
+    for _ in range(args.steps):
+        rc = lib.rtg_render_device(ds.h, C.byref(pk), C.c_void_p(rgb.data_ptr()), None, None,
+                                   C.c_void_p(counts.data_ptr()), sp)
+        if rc != 0:
+            raise RuntimeError(f"rtg_render_device failed: {_capi.last_error()}")
+    stream.synchronize()
+    kt = rt.kernel_times()
+    dom = max(kt, key=lambda k: kt[k][2])
+    launches, items, ms = kt[dom]
 
     # algorithmic traffic of the frame's ray set, canonical oracle BVH, tile sample
     from oracle import pyoracle as po
@@ -115,13 +156,27 @@ def main() -> int:
     rng = np.random.default_rng(11)
     threads = max(1, min(64, os.cpu_count() or 1))
     traffic_tiles = np.sort(rng.choice(tx * ty, size=(tx * ty) // 16, replace=False)).astype(np.int32)
-    tr = osc.traffic(p, threads=threads, tiles=traffic_tiles)
+    tm = traffic_model(osc.traffic(p, threads=threads, tiles=traffic_tiles))
     tiles = traffic_tiles[:: max(1, args.cpu_tiles // 16)]
-    bpr = algorithmic_bytes_per_ray(tr)
-    achieved = rays_per_frame * bpr / (kernel_ms / 1000.0) / 1e9
+    if dom == "k_occlude":
+        unit, per_unit = "traced shadow ray", tm["per_shadow_ray"]
+    elif dom == "k_extend":
+        unit, per_unit = "closest-hit ray", tm["per_closest_ray"]
+    else:  # the megakernel traces the whole frame
+        unit, per_unit = "reference ray", tm["per_reference_ray"]
+        items = rays_per_frame * launches
+    bytes_per_launch = items * per_unit / launches
+    avg_ms = ms / launches
+    achieved = bytes_per_launch / (avg_ms / 1000.0) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "bytes_per_ray": round(bpr, 1), "kernel_ms": round(kernel_ms, 4)}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(eng_name, dom),
+                "kernel": dom, "launches_per_frame": launches // args.steps, "avg_launch_ms": round(avg_ms, 4),
+                "work_unit": unit, "units_per_launch": round(items / launches), "bytes_per_unit": round(per_unit, 1),
+                "frame": {"device_ms": round(frame_ms, 4),
+                          "achieved_GBs": round(rays_per_frame * tm["per_reference_ray"] / frame_ms / 1e6, 1),
+                          "bytes_per_reference_ray": round(tm["per_reference_ray"], 1),
+                          "shadow_rays_skipped_frac": round(tm["shadow_skipped_frac"], 4)},
+                "kernels_ms_per_frame": {k: round(v[2] / args.steps, 4) for k, v in sorted(kt.items())}}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -136,7 +191,7 @@ def main() -> int:
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "1920x1080 1spp depth 4, 200k-tri displaced icosphere + 64 spheres, 4 point lights",
-                   "engine": args.engine, "rays_per_frame": rays_per_frame,
+                   "engine": {"mega": "megakernel", "wave": "wavefront"}[eng_name], "rays_per_frame": rays_per_frame,
                    "rays_by_type": dict(zip(["primary", "shadow", "reflect", "refract"], per_type)),
                    "bvh_build_ms": round(build_ms, 3)},
         "roofline": roofline, "cpu_baseline": cpu,

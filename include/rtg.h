@@ -71,6 +71,7 @@ extern "C" {
 
 /* rtg_params.flags */
 #define RTG_FLAG_FP64_RETEST 0x1 /* re-test near-tie primary candidates in FP64 */
+#define RTG_FLAG_KERNEL_TIMES 0x2 /* time every kernel launch of the render (see rtg_kernel_times) */
 
 /* Hard limits of this build (the reference's own caps are noted beside). */
 #define RTG_MAX_LIGHTS 32      /* reference AddLight caps at 8 (Scene.cpp:85-86) */
@@ -244,6 +245,22 @@ int rtg_cancel(rtg_job_t job);
 int rtg_job_wait(rtg_job_t job, rtg_stats *stats);
 /* Release a job (waits for it first if it is still running). */
 int rtg_job_free(rtg_job_t job);
+
+/* ---- kernel timing ---------------------------------------------------- */
+
+/* Per-kernel device time of the renders issued with RTG_FLAG_KERNEL_TIMES
+ * since the previous call: HIP events recorded on the render's stream around
+ * every launch.  Waits for those events.  `items` counts the work units the
+ * launches processed (rays, shadow rays, tree nodes or pixels).  No
+ * counterpart in the reference, which has no device. */
+typedef struct rtg_kernel_time {
+    char name[24];
+    uint32_t launches;
+    uint32_t pad;
+    uint64_t items;
+    double ms;
+} rtg_kernel_time;
+int rtg_kernel_times(rtg_kernel_time *out, int max, int *n);
 
 /* ---- batched rendering (many requests, one persistent launch) --------- */
 

@@ -103,6 +103,7 @@ typedef struct {
     int max_level;
     oracle_stats st;
     oracle_traffic *traffic; /* non-NULL: count canonical-BVH work per traced ray */
+    int shadow_matters;      /* traffic only: the pending shadow ray can change the colour */
 } octx;
 
 /* ----------------------------------------------------- canonical BVH (FP64) */
@@ -698,6 +699,11 @@ static int count_cb(void *ud, const oprim *p, double *tmax) {
 }
 
 static void count_ray_traffic(octx *cx, const float o[4], const float d[4], double tmax, int anyhit) {
+    if (anyhit && !cx->shadow_matters) {
+        cx->traffic->shadow_skipped++;
+        return;
+    }
+    const uint64_t v0 = cx->traffic->node_visits, t0 = cx->traffic->tri_tests, s0 = cx->traffic->sph_tests;
     cwalk W;
     memset(&W, 0, sizeof(W));
     W.S = cx->S;
@@ -710,6 +716,12 @@ static void count_ray_traffic(octx *cx, const float o[4], const float d[4], doub
     cx->traffic->rays++;
     bvh_walk(cx->S, od, dd, &tm, count_cb, &W, &cx->traffic->node_visits, &cx->traffic->tri_tests,
              &cx->traffic->sph_tests);
+    if (anyhit) {
+        cx->traffic->shadow_rays++;
+        cx->traffic->shadow_node_visits += cx->traffic->node_visits - v0;
+        cx->traffic->shadow_tri_tests += cx->traffic->tri_tests - t0;
+        cx->traffic->shadow_sph_tests += cx->traffic->sph_tests - s0;
+    }
 }
 
 /* ------------------------------------------------------ near-tie band ---- */
@@ -1098,6 +1110,11 @@ static void *worker(void *arg) {
         J->traffic->node_visits += tr.node_visits;
         J->traffic->tri_tests += tr.tri_tests;
         J->traffic->sph_tests += tr.sph_tests;
+        J->traffic->shadow_rays += tr.shadow_rays;
+        J->traffic->shadow_node_visits += tr.shadow_node_visits;
+        J->traffic->shadow_tri_tests += tr.shadow_tri_tests;
+        J->traffic->shadow_sph_tests += tr.shadow_sph_tests;
+        J->traffic->shadow_skipped += tr.shadow_skipped;
     }
     pthread_mutex_unlock(&J->lock);
     return NULL;

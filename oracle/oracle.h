@@ -47,11 +47,17 @@ typedef struct oracle_stats {
 /* Canonical-BVH traffic counts of a frame's ray set (binned SAH with 16 bins,
  * BVH2, <= 4 primitives per leaf, FP64 build). */
 typedef struct oracle_traffic {
+    /* every traced ray: closest-hit rays + shadow rays that can change the
+     * colour (a shadow ray toward a light with N.l <= 0 and N.h <= 0 is not
+     * traced by the device; its outcome cannot change the pixel) */
     uint64_t rays;
     uint64_t node_visits;  /* internal-node box-pair tests */
     uint64_t tri_tests;
     uint64_t sph_tests;
     uint64_t n_nodes, n_prims;
+    /* the shadow (any-hit) part of the above */
+    uint64_t shadow_rays, shadow_node_visits, shadow_tri_tests, shadow_sph_tests;
+    uint64_t shadow_skipped; /* reference shadow rays whose outcome cannot matter */
 } oracle_traffic;
 
 typedef struct oracle_scene_s *oracle_scene_t;

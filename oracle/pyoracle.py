@@ -35,6 +35,8 @@ class oracle_traffic(C.Structure):
     _fields_ = [
         ("rays", C.c_uint64), ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64),
         ("sph_tests", C.c_uint64), ("n_nodes", C.c_uint64), ("n_prims", C.c_uint64),
+        ("shadow_rays", C.c_uint64), ("shadow_node_visits", C.c_uint64), ("shadow_tri_tests", C.c_uint64),
+        ("shadow_sph_tests", C.c_uint64), ("shadow_skipped", C.c_uint64),
     ]
 
 

@@ -19,6 +19,7 @@ import numpy as np
 
 from . import _capi
 from ._capi import (RTG_ENGINE_AUTO, RTG_ENGINE_MEGAKERNEL, RTG_ENGINE_WAVEFRONT, RTG_FLAG_FP64_RETEST,
+                    RTG_FLAG_KERNEL_TIMES,
                     RTG_MODE_RAYTRACE, RtgError, check, rtg_gen_info, rtg_gen_opts, rtg_params, rtg_stats)
 
 __all__ = [
@@ -177,3 +178,14 @@ class DeviceScene:
             self.close()
         except Exception:
             pass
+
+
+def kernel_times() -> dict:
+    """Per-kernel device time of the renders issued with RTG_FLAG_KERNEL_TIMES
+    since the previous call: {name: (launches, items, ms)}."""
+    lib = _capi.load()
+    buf = (_capi.rtg_kernel_time * 16)()
+    n = C.c_int()
+    check(lib.rtg_kernel_times(buf, 16, C.byref(n)), "rtg_kernel_times")
+    return {buf[i].name.decode(): (buf[i].launches, buf[i].items, buf[i].ms) for i in range(min(n.value, 16))}
+

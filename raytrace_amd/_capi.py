@@ -26,6 +26,7 @@ RTG_ENGINE_AUTO = 0
 RTG_ENGINE_MEGAKERNEL = 1
 RTG_ENGINE_WAVEFRONT = 2
 RTG_FLAG_FP64_RETEST = 0x1
+RTG_FLAG_KERNEL_TIMES = 0x2
 
 
 class rtg_vec4(C.Structure):
@@ -103,6 +104,11 @@ class rtg_params(C.Structure):
     ]
 
 
+class rtg_kernel_time(C.Structure):
+    _fields_ = [("name", C.c_char * 24), ("launches", C.c_uint32), ("pad", C.c_uint32), ("items", C.c_uint64),
+                ("ms", C.c_double)]
+
+
 class rtg_stats(C.Structure):
     _fields_ = [
         ("rays_primary", C.c_uint64), ("rays_shadow", C.c_uint64), ("rays_reflect", C.c_uint64),
@@ -157,6 +163,7 @@ SIGNATURES = {
     "rtg_job_wait": (_I, [_P, C.POINTER(rtg_stats)]),
     "rtg_job_free": (_I, [_P]),
     "rtg_render_batch": (_I, [C.POINTER(rtg_request), _I, C.POINTER(rtg_stats)]),
+    "rtg_kernel_times": (_I, [C.POINTER(rtg_kernel_time), _I, C.POINTER(_I)]),
     # rtg_host.h
     "rtg_host_scene_config": (_I, [_I, C.POINTER(rtg_gen_opts), C.POINTER(_P), C.POINTER(rtg_gen_info)]),
     "rtg_host_scene_new": (_I, [C.POINTER(_P)]),

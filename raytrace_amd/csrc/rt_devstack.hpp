@@ -65,24 +65,35 @@ struct alignas(256) LineCtr {
 // chunks share an L2), then from the other XCDs' eighths once its own is
 // exhausted.  `ctr` points to 8 counters that are zero at launch.  Every lane
 // of the wave must call next(); it returns the chunk's first item, or -1
-// when every chunk is taken (each wave stops after at most region + 8 calls).
+// when every chunk is taken.  A region whose counter was seen past its end is
+// never tried again by the wave, so a wave makes at most 8 failed attempts.
 struct WaveQueue {
     LineCtr *ctr;
-    int nchunks, region, x, left;
+    int nchunks, region, x;
+    uint32_t dead;  // regions this wave has seen exhausted
     __device__ WaveQueue(LineCtr *c, int n_items)
         : ctr(c), nchunks((n_items + 63) >> 6), region((((n_items + 63) >> 6) + 7) >> 3), x(blockIdx.x & 7),
-          left(8) {}
+          dead(0) {}
+    __device__ int region_len(int r) const { return max(0, min(region, nchunks - r * region)); }
     __device__ int next() {
-        while (left > 0) {
+        if (nchunks == 0) return -1;
+        const int lane = __lane_id();
+        for (;;) {
             int c = 0;
-            if (__lane_id() == 0) c = (int)atomicAdd(&ctr[x].v, 1u);
+            if (lane == 0) c = (int)atomicAdd(&ctr[x].v, 1u);
             c = __shfl(c, 0, 64);
-            const int chunk = x * region + c;
-            if (c < region && chunk < nchunks) return chunk << 6;
-            x = (x + 1) & 7;
-            --left;
+            if (c < region_len(x)) return (x * region + c) << 6;
+            dead |= 1u << x;
+            // regions with chunks left, read in one round trip
+            bool left = false;
+            if (lane < 8 && !((dead >> lane) & 1u))
+                left = (int)__hip_atomic_load(&ctr[lane].v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+                       region_len(lane);
+            const uint32_t live = (uint32_t)__ballot(left) & 0xFFu & ~dead;
+            if (live == 0) return -1;
+            const uint32_t rot = (live >> x) | (live << (8 - x));  // bit i: region (x + i) & 7
+            x = (x + __builtin_ctz(rot & 0xFFu)) & 7;
         }
-        return -1;
     }
 };
 

@@ -1,5 +1,6 @@
 // rtg_api.hip — implementation of the librtg C ABI (include/rtg.h).
 #include <chrono>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 
@@ -102,10 +103,47 @@ int ensure(DevBuf &b, size_t n) {
 
 Arena *arena() { return g_arena.get(); }
 
+static hipEvent_t take_event(Arena &A) {
+    if (!A.kevents.empty()) {
+        hipEvent_t e = A.kevents.back();
+        A.kevents.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+int ktime_begin(Arena &A, hipStream_t s, int kind, uint64_t items) {
+    if (!A.ktimes) return -1;
+    Arena::KernelRec r{kind, take_event(A), take_event(A), items, -1};
+    if (r.e0 == nullptr || r.e1 == nullptr || hipEventRecord(r.e0, s) != hipSuccess) {
+        if (r.e0) A.kevents.push_back(r.e0);
+        if (r.e1) A.kevents.push_back(r.e1);
+        return -1;
+    }
+    A.krecs.push_back(r);
+    return (int)A.krecs.size() - 1;
+}
+
+void ktime_end(Arena &A, hipStream_t s, int rec, const unsigned *dev_items) {
+    if (rec < 0) return;
+    Arena::KernelRec &r = A.krecs[rec];
+    (void)hipEventRecord(r.e1, s);
+    if (dev_items != nullptr && A.kitems_host != nullptr && A.kitems_next < Arena::kItemSlots) {
+        r.slot = A.kitems_next++;
+        (void)hipMemcpyAsync(A.kitems_host + r.slot, dev_items, sizeof(unsigned), hipMemcpyDeviceToHost, s);
+    }
+}
+
 // Render one frame into device outputs; the caller holds A.lock.
 static int render_device(SceneRes &S, const rtg_params &p, int W, int H, const RenderOut &o, hipStream_t stream,
                          Arena &A) {
     const size_t npix = (size_t)W * H;
+    A.ktimes = (p.flags & RTG_FLAG_KERNEL_TIMES) != 0;
+    if (A.ktimes && A.kitems_host == nullptr)
+        RTG_HIP(hipHostMalloc(reinterpret_cast<void **>(&A.kitems_host), Arena::kItemSlots * sizeof(unsigned),
+                              hipHostMallocDefault));
     k_fill_u8<<<1024, 256, 0, stream>>>(o.rgb, npix * 3, 127);
     if (o.ids) k_fill_u8<<<1024, 256, 0, stream>>>(reinterpret_cast<uint8_t *>(o.ids), npix * 8, 0xff);
     k_zero_u64<<<1, 64, 0, stream>>>(o.counts, 4);
@@ -177,6 +215,12 @@ int rtg_shutdown(void) {
         A.out_ids.release();
         A.out_t.release();
         (void)hipHostFree(A.cancel_host);
+        if (A.qcount_host) (void)hipHostFree(A.qcount_host);
+        if (A.kitems_host) (void)hipHostFree(A.kitems_host);
+        for (const Arena::KernelRec &r : A.krecs) A.kevents.push_back(r.e0), A.kevents.push_back(r.e1);
+        for (hipEvent_t e : A.kevents) (void)hipEventDestroy(e);
+        A.krecs.clear();
+        A.kevents.clear();
         (void)hipEventDestroy(A.ev0);
         (void)hipEventDestroy(A.ev1);
         (void)hipStreamDestroy(A.stream);
@@ -370,6 +414,41 @@ int rtg_job_free(rtg_job_t job) {
     if (job->th.joinable()) job->th.join();
     delete job;
     return RTG_OK;
+}
+
+int rtg_kernel_times(rtg_kernel_time *out, int max, int *n) {
+    if (n == nullptr || (out == nullptr && max > 0)) return fail(RTG_E_INVALID, "NULL argument");
+    if (!arena()) return fail(RTG_E_STATE, "call rtg_init first");
+    Arena &A = *arena();
+    std::lock_guard<std::mutex> l(A.lock);
+    static const char *names[KT_COUNT] = {"k_extend", "k_shade", "k_occlude", "k_combine", "k_resolve", "k_megakernel"};
+    rtg_kernel_time acc[KT_COUNT];
+    memset(acc, 0, sizeof(acc));
+    int st = RTG_OK;
+    for (const Arena::KernelRec &r : A.krecs) {
+        float ms = 0.0f;
+        if (hipEventSynchronize(r.e1) != hipSuccess || hipEventElapsedTime(&ms, r.e0, r.e1) != hipSuccess)
+            st = fail(RTG_E_DEVICE, "kernel timing events");
+        rtg_kernel_time &a = acc[r.kind];
+        a.launches++;
+        a.items += r.slot >= 0 ? A.kitems_host[r.slot] : r.items;
+        a.ms += ms;
+        A.kevents.push_back(r.e0);
+        A.kevents.push_back(r.e1);
+    }
+    A.krecs.clear();
+    A.kitems_next = 0;
+    int k = 0;
+    for (int i = 0; i < KT_COUNT; ++i) {
+        if (acc[i].launches == 0) continue;
+        if (k < max) {
+            out[k] = acc[i];
+            snprintf(out[k].name, sizeof(out[k].name), "%s", names[i]);
+        }
+        ++k;
+    }
+    *n = k;
+    return st;
 }
 
 int rtg_render_batch(const rtg_request *reqs, int n, rtg_stats *stats) {

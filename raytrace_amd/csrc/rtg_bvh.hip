@@ -296,6 +296,22 @@ int scene_build(SceneRes &S, const rtg_scene_desc &d, hipStream_t stream) {
     S.lights_on = 0;
     for (int i = 0; i < d.n_lights; ++i)
         if (d.lights[i].on) S.lights_on |= 1u << i;
+    // Most child rays one hit can spawn (reflection and/or refraction) over
+    // the materials of shown objects: bounds the wavefront queues per level.
+    auto fan_of = [&](int m) {
+        if (m < 0 || m >= d.n_materials) return 0;
+        return (d.materials[m].reflect > 0.01f ? 1 : 0) + (d.materials[m].refract > 0.01f ? 1 : 0);
+    };
+    S.max_fanout = 0;
+    for (int i = 0; i < d.n_objects; ++i) {
+        const rtg_object &o = d.objects[i];
+        if (!o.show) continue;
+        if (o.type == RTG_OBJ_MODEL) {
+            for (int k = 0; k < o.part_count; ++k) S.max_fanout = std::max(S.max_fanout, fan_of(d.parts[o.part_begin + k].mtl));
+        } else {
+            S.max_fanout = std::max(S.max_fanout, fan_of(o.mtl));
+        }
+    }
 
     SceneView &v = S.view;
     v = SceneView{};

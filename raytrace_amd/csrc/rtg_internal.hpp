@@ -46,6 +46,7 @@ struct SceneRes {
     size_t bytes = 0;
     int n_nodes = 0;
     uint32_t lights_on = 0;  // bit i: light i is switched on
+    int max_fanout = 2;      // most child rays per hit over the shown objects' materials
 };
 
 // Per-device render state: stream, spill stacks, tables, counters, outputs.
@@ -63,8 +64,29 @@ struct Arena {
     DevBuf w_rays[kLevels], w_hits[kLevels], w_nodes[kLevels], w_masks[kLevels], w_slots[kLevels],
         w_shadow[kLevels], w_samples, w_qcount;
     unsigned *qcount_host = nullptr;  // pinned
+    // kernel timing (RTG_FLAG_KERNEL_TIMES)
+    struct KernelRec {
+        int kind;
+        hipEvent_t e0, e1;
+        uint64_t items;
+        int slot;  // >= 0: item count read back from the device into kitems_host[slot]
+    };
+    bool ktimes = false;
+    std::vector<KernelRec> krecs;
+    std::vector<hipEvent_t> kevents;  // free timing events
+    unsigned *kitems_host = nullptr;  // pinned
+    int kitems_next = 0;
+    static constexpr int kItemSlots = 4096;
     std::mutex lock;  // one render at a time on the arena
 };
+
+enum KernelKind { KT_EXTEND, KT_SHADE, KT_OCCLUDE, KT_COMBINE, KT_RESOLVE, KT_MEGA, KT_COUNT };
+// Start timing a launch of `kind` on `s` (no-op unless A.ktimes); returns a
+// record index for ktime_end, or -1.
+int ktime_begin(Arena &A, hipStream_t s, int kind, uint64_t items);
+// End the record; `dev_items` (optional) is a device counter whose value at
+// this point of the stream is the launch's item count.
+void ktime_end(Arena &A, hipStream_t s, int rec, const unsigned *dev_items = nullptr);
 
 int upload_raygen_tables(const SceneRes &S, int W, int H, hipStream_t stream, Arena &A, double &dp);
 int persistent_blocks(const Arena &A);

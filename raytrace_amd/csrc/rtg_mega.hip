@@ -128,7 +128,9 @@ int launch_megakernel(const SceneRes &S, const rtg_params &p, int W, int H, cons
     a.counts = o.counts;
     a.cancel = A.cancel_dev;
     a.spill = A.spill.as<int>();
+    const int kt = ktime_begin(A, stream, KT_MEGA, (uint64_t)W * H * a.spp);
     k_megakernel<<<blocks, kBlockThreads, 0, stream>>>(S.view, a);
+    ktime_end(A, stream, kt);
     RTG_HIP(hipGetLastError());
     return RTG_OK;
 }

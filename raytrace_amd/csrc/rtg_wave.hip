@@ -350,6 +350,10 @@ __global__ void k_zero_u32(unsigned *p, int n) {
     if (i < n) p[i] = 0;
 }
 
+// Largest queue memory (bytes) a frame may reserve to run without host round
+// trips between levels.
+constexpr size_t kNoSyncBudget = size_t(8) << 30;
+
 // Resident blocks per CU of a kernel at kBlockThreads threads (registers and
 // LDS permitting), queried once.
 template <class K>
@@ -390,11 +394,26 @@ int launch_wavefront(const SceneRes &S, const rtg_params &p, int W, int H, const
     const int ctr_words = Arena::kLevels * kCtrWords;
     k_zero_u32<<<(ctr_words + 255) / 256, 256, 0, stream>>>(A.w_qcount.as<unsigned>(), ctr_words);
     auto grid = [](int g, size_t items) { return (int)std::max<size_t>(8, std::min<size_t>(g, (items + 127) / 128)); };
+    // Queue capacity per level: level k holds at most n_samples * fanout^k
+    // rays.  When those bounds fit the budget, every queue is sized by its
+    // bound and the frame is enqueued without a host round trip (kernels read
+    // the live counts from the device); otherwise each level is sized from
+    // the previous level's count, read back to the host.
+    const size_t per_ray = sizeof(WRay) + sizeof(WHit) + sizeof(WNode) + sizeof(unsigned) + 2 * sizeof(float4) +
+                           (size_t)n_on * sizeof(unsigned);
+    size_t bound[Arena::kLevels] = {};
+    size_t need = 0;
+    bool no_sync = true;
+    for (int k = 0; k <= p.max_level; ++k) {
+        bound[k] = k == 0 ? (size_t)a.n_samples : bound[k - 1] * (size_t)S.max_fanout;
+        need += bound[k] * per_ray;
+        if (need > kNoSyncBudget) no_sync = false;
+    }
     size_t n_k = (size_t)a.n_samples;
     int last = -1;
     for (int k = 0; k <= p.max_level && n_k > 0; ++k) {
         if (*A.cancel_host) return RTG_OK;
-        const size_t cap_next = k < p.max_level ? 2 * n_k : 1;
+        const size_t cap_next = k < p.max_level ? (size_t)std::max(1, S.max_fanout) * n_k : 1;
         if (ensure(A.w_hits[k], n_k * sizeof(WHit)) || ensure(A.w_nodes[k], n_k * sizeof(WNode)) ||
             ensure(A.w_masks[k], n_k * sizeof(unsigned)) || ensure(A.w_slots[k], 2 * n_k * sizeof(float4)) ||
             ensure(A.w_shadow[k], std::max<size_t>(1, n_k * n_on) * sizeof(unsigned)) ||
@@ -403,19 +422,30 @@ int launch_wavefront(const SceneRes &S, const rtg_params &p, int W, int H, const
         const WRay *rays = k == 0 ? nullptr : A.w_rays[k].as<WRay>();
         const unsigned *n_ptr = k == 0 ? nullptr : &ctr[k - 1].n_next.v;
         WRay *next = A.w_rays[k + 1 < Arena::kLevels ? k + 1 : k].as<WRay>();
+        int kt = ktime_begin(A, stream, KT_EXTEND, n_k);
         k_extend<<<grid(g_extend, n_k), kBlockThreads, 0, stream>>>(S.view, a, k, rays, n_ptr,
                                                                     A.w_hits[k].as<WHit>(), ctr[k].sched[0]);
+        ktime_end(A, stream, kt);
+        kt = ktime_begin(A, stream, KT_SHADE, n_k);
         k_shade<<<grid(g_shade, n_k), kBlockThreads, 0, stream>>>(
             S.view, a, k, rays, n_ptr, A.w_hits[k].as<WHit>(), A.w_nodes[k].as<WNode>(), A.w_masks[k].as<unsigned>(),
             next, ctr + k, A.w_shadow[k].as<unsigned>(), lights_on);
+        ktime_end(A, stream, kt);
         k_fill_slots<<<256, 256, 0, stream>>>(A.w_slots[k].as<float4>(), n_ptr, k == 0 ? a.n_samples : -1);
-        if (n_on > 0)
+        if (n_on > 0) {
+            kt = ktime_begin(A, stream, KT_OCCLUDE, 0);
             k_occlude<<<grid(g_occlude, n_k * n_on), kBlockThreads, 0, stream>>>(
                 S.view, a, A.w_nodes[k].as<WNode>(), A.w_shadow[k].as<unsigned>(), ctr + k,
                 A.w_masks[k].as<unsigned>());
+            ktime_end(A, stream, kt, &ctr[k].n_shadow.v);
+        }
         RTG_HIP(hipGetLastError());
         last = k;
         if (k == p.max_level) break;
+        if (no_sync) {
+            n_k = bound[k + 1];
+            continue;
+        }
         RTG_HIP(hipMemcpyAsync(A.qcount_host, &ctr[k].n_next.v, sizeof(unsigned), hipMemcpyDeviceToHost, stream));
         RTG_HIP(hipStreamSynchronize(stream));
         n_k = A.qcount_host[0];
@@ -426,11 +456,15 @@ int launch_wavefront(const SceneRes &S, const rtg_params &p, int W, int H, const
     for (int k = last; k >= 0; --k) {
         const WRay *rays = k == 0 ? nullptr : A.w_rays[k].as<WRay>();
         const unsigned *n_ptr = k == 0 ? nullptr : &ctr[k - 1].n_next.v;
+        const int kt = ktime_begin(A, stream, KT_COMBINE, k == 0 ? (uint64_t)a.n_samples : 0);
         k_combine<<<blocks, kBlockThreads, 0, stream>>>(
             S.view, a, k, rays, n_ptr, A.w_nodes[k].as<WNode>(), A.w_masks[k].as<unsigned>(),
             A.w_slots[k].as<float4>(), k > 0 ? A.w_slots[k - 1].as<float4>() : nullptr, A.w_samples.as<float4>());
+        ktime_end(A, stream, kt, n_ptr);
     }
+    const int kt = ktime_begin(A, stream, KT_RESOLVE, (uint64_t)W * H);
     k_resolve<<<1024, 256, 0, stream>>>(a, A.w_samples.as<float4>(), o.rgb);
+    ktime_end(A, stream, kt);
     RTG_HIP(hipGetLastError());
     return RTG_OK;
 }
