@@ -72,6 +72,7 @@ extern "C" {
 /* rtg_params.flags */
 #define RTG_FLAG_FP64_RETEST 0x1 /* re-test near-tie primary candidates in FP64 */
 #define RTG_FLAG_KERNEL_TIMES 0x2 /* time every kernel launch of the render (see rtg_kernel_times) */
+#define RTG_FLAG_EAGER 0x4        /* batch: launch kernel by kernel, no captured graph */
 
 /* Hard limits of this build (the reference's own caps are noted beside). */
 #define RTG_MAX_LIGHTS 32      /* reference AddLight caps at 8 (Scene.cpp:85-86) */
@@ -181,6 +182,7 @@ typedef struct rtg_stats {
     double render_ms; /* device time of the render, HIP events */
     double build_ms;  /* device time of the last BVH build for this scene */
     uint64_t retests_fp64;
+    uint64_t graph_replays; /* batch: 1 when the launch sequence was replayed from a captured hipGraph */
 } rtg_stats;
 
 typedef struct rtg_scene_s *rtg_scene_t;
@@ -271,10 +273,13 @@ typedef struct rtg_request {
     int32_t *ids;   /* optional host W*H*2 */
 } rtg_request;
 
-/* Render n requests of uneven size and scene together: all their rays share
- * one global queue per depth, processed by persistent kernels, replayed from
- * a captured hipGraph when the batch shape repeats.  Each request's result is
- * identical to rendering it alone with rtg_render. */
+/* Render n requests (n <= 4096) of uneven size and scene together, as one
+ * wavefront: all their rays share one global queue per depth, processed by
+ * persistent kernels; the launch sequence is captured into a hipGraph and
+ * replayed for later batches of the same padded shape (unless a request sets
+ * RTG_FLAG_EAGER).  Each request's result is identical to rendering it alone
+ * with rtg_render.  `stats` sums the ray counts of all requests.  No
+ * counterpart in the reference, which renders one frame per start(). */
 int rtg_render_batch(const rtg_request *reqs, int n, rtg_stats *stats);
 
 #ifdef __cplusplus

@@ -189,3 +189,27 @@ def kernel_times() -> dict:
     check(lib.rtg_kernel_times(buf, 16, C.byref(n)), "rtg_kernel_times")
     return {buf[i].name.decode(): (buf[i].launches, buf[i].items, buf[i].ms) for i in range(min(n.value, 16))}
 
+
+
+def render_batch(items, ids: bool = True, eager: bool = False):
+    """rtg_render_batch over [(DeviceScene, rtg_params), ...]: one wavefront for
+    all requests.  Returns ([{"rgb", "ids"} per request], rtg_stats)."""
+    lib = _capi.load()
+    n = len(items)
+    reqs = (_capi.rtg_request * max(1, n))()
+    outs = []
+    for k, (ds, p) in enumerate(items):
+        q = rtg_params()
+        C.memmove(C.byref(q), C.byref(p), C.sizeof(rtg_params))
+        if eager:
+            q.flags |= _capi.RTG_FLAG_EAGER
+        rgb = np.empty((p.height, p.width, 3), np.uint8)
+        idb = np.empty((p.height, p.width, 2), np.int32) if ids else None
+        reqs[k].scene = ds.h
+        reqs[k].params = q
+        reqs[k].rgb = rgb.ctypes.data_as(C.POINTER(C.c_uint8))
+        reqs[k].ids = idb.ctypes.data_as(C.POINTER(C.c_int32)) if ids else None
+        outs.append({"rgb": rgb, "ids": idb})
+    st = rtg_stats()
+    check(lib.rtg_render_batch(reqs, n, C.byref(st)), "rtg_render_batch")
+    return outs, st

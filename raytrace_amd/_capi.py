@@ -27,6 +27,7 @@ RTG_ENGINE_MEGAKERNEL = 1
 RTG_ENGINE_WAVEFRONT = 2
 RTG_FLAG_FP64_RETEST = 0x1
 RTG_FLAG_KERNEL_TIMES = 0x2
+RTG_FLAG_EAGER = 0x4
 
 
 class rtg_vec4(C.Structure):
@@ -113,7 +114,7 @@ class rtg_stats(C.Structure):
     _fields_ = [
         ("rays_primary", C.c_uint64), ("rays_shadow", C.c_uint64), ("rays_reflect", C.c_uint64),
         ("rays_refract", C.c_uint64), ("rays_total", C.c_uint64), ("render_ms", C.c_double),
-        ("build_ms", C.c_double), ("retests_fp64", C.c_uint64),
+        ("build_ms", C.c_double), ("retests_fp64", C.c_uint64), ("graph_replays", C.c_uint64),
     ]
 
 

@@ -118,4 +118,15 @@ inline std::string prepare_scene(const rtg_scene_desc &d, ScenePrep &P) {
     return "";
 }
 
+// Level-0 length of a batch of `total` samples, padded so that batches of
+// similar size share one captured graph: 4096, then +12.5% steps rounded up
+// to a multiple of 4096.  Always a whole number of 64-sample chunks (the
+// batch's chunk -> request table has one entry per chunk) and at most 12.5%
+// (or 4096 samples) of padding.
+inline size_t batch_bucket(size_t total) {
+    size_t padded = 4096;
+    while (padded < total) padded = (padded + padded / 8 + 4095) & ~size_t(4095);
+    return padded;
+}
+
 }  // namespace rtg

@@ -76,6 +76,7 @@ struct SceneView {
     const BvhNode *nodes;
     const float4 *prims;
     int root;  // child encoding of the root; 0 with n_prims == 0 means empty
+    uint32_t lights_on;  // bit i: light i is switched on (Light::bLight)
     int n_prims;
     const DevObject *objs;
     int n_objs;

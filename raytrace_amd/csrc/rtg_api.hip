@@ -56,6 +56,7 @@ int validate_desc(const rtg_scene_desc *d) {
         (d->n_lights && !d->lights) || (d->n_textures && !d->textures))
         return fail(RTG_E_INVALID, "NULL array in scene descriptor");
     if (d->n_lights > RTG_MAX_LIGHTS) return fail(RTG_E_LIMIT, "more than RTG_MAX_LIGHTS lights");
+    if (d->n_materials > (1 << 20)) return fail(RTG_E_LIMIT, "more than 2^20 materials");
     for (int i = 0; i < d->n_materials; ++i)
         if (d->materials[i].tex >= d->n_textures) return fail(RTG_E_INVALID, "material texture out of range");
     for (int i = 0; i < d->n_parts; ++i)
@@ -99,6 +100,42 @@ void DevBuf::release() {
 int ensure(DevBuf &b, size_t n) {
     if (b.p != nullptr && b.bytes >= n) return RTG_OK;
     return b.alloc(n);
+}
+
+int Workspace::grow(DevBuf &b, size_t n) {
+    if (b.p != nullptr && b.bytes >= n) return RTG_OK;
+    ++epoch;
+    return b.alloc(n + n / 4);  // headroom: fewer moves, fewer graph re-captures
+}
+
+int Workspace::order_before(hipStream_t s) {
+    if (done != nullptr) RTG_HIP(hipStreamWaitEvent(s, done, 0));
+    return RTG_OK;
+}
+
+int Workspace::mark_done(hipStream_t s) {
+    if (done == nullptr) RTG_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+    RTG_HIP(hipEventRecord(done, s));
+    return RTG_OK;
+}
+
+void Workspace::release() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    graph = nullptr;
+    if (done) (void)hipEventDestroy(done);
+    done = nullptr;
+    spill.release(), tables.release(), w_samples.release(), w_qcount.release();
+    b_reqs.release(), b_scenes.release(), b_chunks.release();
+    for (int k = 0; k < kLevels; ++k) {
+        w_rays[k].release(), w_hits[k].release(), w_nodes[k].release();
+        w_masks[k].release(), w_slots[k].release(), w_shadow[k].release();
+    }
+    if (qcount_host) (void)hipHostFree(qcount_host);
+    if (staging) (void)hipHostFree(staging);
+    qcount_host = nullptr;
+    staging = nullptr;
+    staging_bytes = 0;
+    ++epoch;
 }
 
 Arena *arena() { return g_arena.get(); }
@@ -152,8 +189,12 @@ static int render_device(SceneRes &S, const rtg_params &p, int W, int H, const R
     // AUTO: the wavefront engine (faster on every measured workload; the
     // megakernel's per-thread ray trees diverge on deep refraction trees).
     int engine = p.engine == RTG_ENGINE_AUTO ? RTG_ENGINE_WAVEFRONT : p.engine;
-    if (engine == RTG_ENGINE_WAVEFRONT) return launch_wavefront(S, p, W, H, o, stream, A);
-    return launch_megakernel(S, p, W, H, o, stream, A);
+    int st = A.ws.order_before(stream);
+    if (st != RTG_OK) return st;
+    st = engine == RTG_ENGINE_WAVEFRONT ? launch_wavefront(S, p, W, H, o, stream, A)
+                                        : launch_megakernel(S, p, W, H, o, stream, A);
+    const int st2 = A.ws.mark_done(stream);
+    return st != RTG_OK ? st : st2;
 }
 
 }  // namespace rtg
@@ -208,14 +249,12 @@ int rtg_shutdown(void) {
     {
         std::lock_guard<std::mutex> l(A.lock);
         (void)hipStreamSynchronize(A.stream);
-        A.spill.release();
-        A.tables.release();
+        A.ws.release();
         A.counters.release();
         A.out_rgb.release();
         A.out_ids.release();
         A.out_t.release();
         (void)hipHostFree(A.cancel_host);
-        if (A.qcount_host) (void)hipHostFree(A.qcount_host);
         if (A.kitems_host) (void)hipHostFree(A.kitems_host);
         for (const Arena::KernelRec &r : A.krecs) A.kevents.push_back(r.e0), A.kevents.push_back(r.e1);
         for (hipEvent_t e : A.kevents) (void)hipEventDestroy(e);
@@ -453,19 +492,71 @@ int rtg_kernel_times(rtg_kernel_time *out, int max, int *n) {
 
 int rtg_render_batch(const rtg_request *reqs, int n, rtg_stats *stats) {
     if (n < 0 || (n > 0 && reqs == nullptr)) return fail(RTG_E_INVALID, "bad request list");
-    rtg_stats total{};
+    if (n > 4096) return fail(RTG_E_LIMIT, "more than 4096 requests in a batch");
+    if (!arena()) return fail(RTG_E_STATE, "call rtg_init first");
+    if (stats) *stats = rtg_stats{};
+    if (n == 0) return RTG_OK;
+    Arena &A = *arena();
+    std::vector<BatchItem> items(n);
+    std::vector<size_t> off_rgb(n), off_ids(n);
+    size_t rgb_bytes = 0, ids_bytes = 0;
+    bool eager = false;
     for (int i = 0; i < n; ++i) {
-        rtg_stats s{};
-        int st = rtg_render(reqs[i].scene, &reqs[i].params, reqs[i].rgb, reqs[i].ids, nullptr, &s);
+        const rtg_request &q = reqs[i];
+        if (q.scene == nullptr || q.rgb == nullptr) return fail(RTG_E_INVALID, "request without scene or rgb");
+        int W, H;
+        int st = resolve_size(q.scene->res, &q.params, W, H);
         if (st != RTG_OK) return st;
-        total.rays_primary += s.rays_primary;
-        total.rays_shadow += s.rays_shadow;
-        total.rays_reflect += s.rays_reflect;
-        total.rays_refract += s.rays_refract;
-        total.rays_total += s.rays_total;
-        total.render_ms += s.render_ms;
+        if (q.params.mode != RTG_MODE_RAYTRACE && q.params.mode != RTG_MODE_REFRACTTEST && q.params.mode != 0)
+            return fail(RTG_E_INVALID, "render mode not supported by this build");
+        eager = eager || (q.params.flags & RTG_FLAG_EAGER) != 0;
+        items[i] = BatchItem{&q.scene->res, q.params, W, H, nullptr, nullptr, nullptr};
+        off_rgb[i] = rgb_bytes;
+        rgb_bytes += ((size_t)W * H * 3 + 255) & ~size_t(255);
+        off_ids[i] = ids_bytes;
+        if (q.ids) ids_bytes += (size_t)W * H * 8;
     }
-    if (stats) *stats = total;
+    std::lock_guard<std::mutex> l(A.lock);
+    A.ktimes = false;
+    if (ensure(A.out_rgb, rgb_bytes) || (ids_bytes && ensure(A.out_ids, ids_bytes)))
+        return fail(RTG_E_NOMEM, "batch output buffers");
+    for (int i = 0; i < n; ++i) {
+        items[i].rgb = A.out_rgb.as<uint8_t>() + off_rgb[i];
+        if (reqs[i].ids) items[i].ids = reinterpret_cast<int32_t *>(A.out_ids.as<uint8_t>() + off_ids[i]);
+    }
+    hipStream_t s = A.stream;
+    unsigned long long *counts = A.counters.as<unsigned long long>();
+    k_fill_u8<<<1024, 256, 0, s>>>(A.out_rgb.as<uint8_t>(), rgb_bytes, 127);
+    if (ids_bytes) k_fill_u8<<<1024, 256, 0, s>>>(A.out_ids.as<uint8_t>(), ids_bytes, 0xff);
+    k_zero_u64<<<1, 64, 0, s>>>(counts, 4);
+    RTG_HIP(hipEventRecord(A.ev0, s));
+    int replayed = 0;
+    int st = A.ws.order_before(s);
+    if (st == RTG_OK) st = launch_batch(items.data(), n, counts, s, A.ws, A, !eager, &replayed);
+    if (st == RTG_OK) st = A.ws.mark_done(s);
+    RTG_HIP(hipEventRecord(A.ev1, s));
+    unsigned long long cnt[4] = {0, 0, 0, 0};
+    if (st == RTG_OK) {
+        for (int i = 0; i < n; ++i) {
+            const size_t npix = (size_t)items[i].W * items[i].H;
+            RTG_HIP(hipMemcpyAsync(reqs[i].rgb, items[i].rgb, npix * 3, hipMemcpyDeviceToHost, s));
+            if (reqs[i].ids) RTG_HIP(hipMemcpyAsync(reqs[i].ids, items[i].ids, npix * 8, hipMemcpyDeviceToHost, s));
+        }
+        RTG_HIP(hipMemcpyAsync(cnt, counts, sizeof(cnt), hipMemcpyDeviceToHost, s));
+    }
+    RTG_HIP(hipStreamSynchronize(s));
+    if (st != RTG_OK) return st;
+    float ms = 0;
+    RTG_HIP(hipEventElapsedTime(&ms, A.ev0, A.ev1));
+    if (stats) {
+        stats->rays_primary = cnt[0];
+        stats->rays_shadow = cnt[1];
+        stats->rays_reflect = cnt[2];
+        stats->rays_refract = cnt[3];
+        stats->rays_total = cnt[0] + cnt[1] + cnt[2] + cnt[3];
+        stats->render_ms = ms;
+        stats->graph_replays = (uint64_t)replayed;
+    }
     return RTG_OK;
 }
 

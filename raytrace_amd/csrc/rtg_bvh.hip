@@ -318,6 +318,7 @@ int scene_build(SceneRes &S, const rtg_scene_desc &d, hipStream_t stream) {
     v.nodes = S.nodes.as<BvhNode>();
     v.prims = S.prims.as<float4>();
     v.root = root;
+    v.lights_on = S.lights_on;
     v.n_prims = n;
     v.objs = S.objs.as<DevObject>();
     v.n_objs = d.n_objects;

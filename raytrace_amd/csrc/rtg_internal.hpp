@@ -49,21 +49,46 @@ struct SceneRes {
     int max_fanout = 2;      // most child rays per hit over the shown objects' materials
 };
 
-// Per-device render state: stream, spill stacks, tables, counters, outputs.
+// Device memory of one in-flight render (or batch): traversal spill stacks,
+// ray-generation tables, wavefront queues and counters, batch tables.  A
+// workspace serves one render at a time; concurrent renders (serving slots)
+// each have their own.
+struct Workspace {
+    static constexpr int kLevels = RTG_MAX_LEVEL + 1;
+    DevBuf spill, tables;
+    uint64_t tables_key = 0;
+    DevBuf w_rays[kLevels], w_hits[kLevels], w_nodes[kLevels], w_masks[kLevels], w_slots[kLevels],
+        w_shadow[kLevels], w_samples, w_qcount;
+    DevBuf b_reqs, b_scenes, b_chunks;  // batch: request table, scene table, chunk -> request
+    unsigned *qcount_host = nullptr;    // pinned
+    void *staging = nullptr;            // pinned upload staging (batch tables)
+    size_t staging_bytes = 0;
+    uint64_t epoch = 0;  // bumped when a buffer moves (captured graphs hold addresses)
+    hipGraphExec_t graph = nullptr;
+    uint64_t graph_key = 0;
+    // Recorded after the last render enqueued with this workspace: a render
+    // on another stream waits for it (device-side) before reusing the buffers.
+    hipEvent_t done = nullptr;
+    int order_before(hipStream_t s);  // make `s` wait for the previous render
+    int mark_done(hipStream_t s);     // record `done` on `s`
+    // Grow `b` to hold `bytes`; bumps the epoch if it moved.
+    int grow(DevBuf &b, size_t bytes);
+    void release();  // frees everything (the stream using it must be idle)
+    ~Workspace() { release(); }
+};
+
+// Per-device render state: default stream and workspace, outputs, cancel
+// flag, kernel timing.
 struct Arena {
+    static constexpr int kLevels = Workspace::kLevels;
     int device = -1;
     int cus = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    DevBuf spill, tables, counters, out_rgb, out_ids, out_t;
-    uint64_t tables_key = 0;
+    DevBuf counters, out_rgb, out_ids, out_t;
+    Workspace ws;
     int *cancel_host = nullptr;  // pinned, mapped
     int *cancel_dev = nullptr;
-    // wavefront engine: per-level queues and records, grown on demand
-    static constexpr int kLevels = RTG_MAX_LEVEL + 1;
-    DevBuf w_rays[kLevels], w_hits[kLevels], w_nodes[kLevels], w_masks[kLevels], w_slots[kLevels],
-        w_shadow[kLevels], w_samples, w_qcount;
-    unsigned *qcount_host = nullptr;  // pinned
     // kernel timing (RTG_FLAG_KERNEL_TIMES)
     struct KernelRec {
         int kind;
@@ -88,7 +113,10 @@ int ktime_begin(Arena &A, hipStream_t s, int kind, uint64_t items);
 // this point of the stream is the launch's item count.
 void ktime_end(Arena &A, hipStream_t s, int rec, const unsigned *dev_items = nullptr);
 
-int upload_raygen_tables(const SceneRes &S, int W, int H, hipStream_t stream, Arena &A, double &dp);
+int upload_raygen_tables(const SceneRes &S, int W, int H, hipStream_t stream, Workspace &ws, double &dp);
+// Ray-generation tables of a W x H frame (RayTracer.cpp:13-27): a[x] =
+// (float)((x - W/2) * dp) then b[y] likewise, dp = tan(fovy*PI/360)/(H/2).
+double raygen_tables(float fovy, int W, int H, float *tab);
 int persistent_blocks(const Arena &A);
 
 Arena *arena();
@@ -109,5 +137,23 @@ int launch_megakernel(const SceneRes &S, const rtg_params &p, int W, int H, cons
 // Launch one frame of the wavefront engine on `stream` (asynchronous).
 int launch_wavefront(const SceneRes &S, const rtg_params &p, int W, int H, const RenderOut &o, hipStream_t stream,
                      Arena &A);
+
+// One request of a batch (rtg_render_batch): a frame of `scene` into device
+// outputs.
+struct BatchItem {
+    const SceneRes *scene;
+    rtg_params params;
+    int W, H;
+    uint8_t *rgb;   // device
+    int32_t *ids;   // device, optional
+    float *tprim;   // device, optional
+};
+// Enqueue every request of a batch on `stream` as one wavefront: all their
+// rays share one queue per level; no host round trip when the queue bounds fit
+// the workspace budget.  With `use_graph`, the launch sequence is captured
+// into a hipGraph and replayed for later batches of the same shape.
+// `counts` (device, 4 x u64) accumulates the ray counts of the whole batch.
+int launch_batch(const BatchItem *items, int n, unsigned long long *counts, hipStream_t stream, Workspace &ws,
+                 Arena &A, bool use_graph, int *graph_replayed);
 
 }  // namespace rtg

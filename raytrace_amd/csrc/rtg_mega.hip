@@ -92,18 +92,24 @@ __global__ __launch_bounds__(kBlockThreads) void k_megakernel(SceneView S, MegaA
 // Ray-generation tables (float)(xcur * dp), (float)(ycur * dp) with
 // dp = tan(fovy * PI / 360) / (H / 2) in double (RayTracer.cpp:14,26).
 // Uploaded once per (W, H, fovy); later frames reuse them.
-int upload_raygen_tables(const SceneRes &S, int W, int H, hipStream_t stream, Arena &A, double &dp) {
+double raygen_tables(float fovy, int W, int H, float *tab) {
+    const double dp = tan((double)fovy * 3.1415926535897932384 / 360) / (H / 2);
+    for (int x = 0; x < W; ++x) tab[x] = (float)((x - W / 2) * dp);
+    for (int y = 0; y < H; ++y) tab[W + y] = (float)((y - H / 2) * dp);
+    return dp;
+}
+
+int upload_raygen_tables(const SceneRes &S, int W, int H, hipStream_t stream, Workspace &ws, double &dp) {
     dp = tan((double)S.camera.fovy * 3.1415926535897932384 / 360) / (H / 2);
     const uint32_t fb = __builtin_bit_cast(uint32_t, S.camera.fovy);
     const uint64_t key = (uint64_t)W | ((uint64_t)H << 16) | ((uint64_t)fb << 32);
-    if (A.tables.p != nullptr && A.tables_key == key) return RTG_OK;
+    if (ws.tables.p != nullptr && ws.tables_key == key) return RTG_OK;
     std::vector<float> tab((size_t)W + H);
-    for (int x = 0; x < W; ++x) tab[x] = (float)((x - W / 2) * dp);
-    for (int y = 0; y < H; ++y) tab[W + y] = (float)((y - H / 2) * dp);
-    if (ensure(A.tables, tab.size() * sizeof(float))) return fail(RTG_E_NOMEM, "tables");
+    dp = raygen_tables(S.camera.fovy, W, H, tab.data());
     RTG_HIP(hipStreamSynchronize(stream));  // earlier frames may still read the old tables
-    RTG_HIP(hipMemcpy(A.tables.p, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
-    A.tables_key = key;
+    if (ws.grow(ws.tables, tab.size() * sizeof(float))) return fail(RTG_E_NOMEM, "tables");
+    RTG_HIP(hipMemcpy(ws.tables.p, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
+    ws.tables_key = key;
     return RTG_OK;
 }
 
@@ -112,22 +118,22 @@ int persistent_blocks(const Arena &A) { return std::max(8, (A.cus > 0 ? A.cus : 
 int launch_megakernel(const SceneRes &S, const rtg_params &p, int W, int H, const RenderOut &o, hipStream_t stream,
                       Arena &A) {
     double dp;
-    int st = upload_raygen_tables(S, W, H, stream, A, dp);
+    int st = upload_raygen_tables(S, W, H, stream, A.ws, dp);
     if (st != RTG_OK) return st;
     const int blocks = persistent_blocks(A);
-    if (ensure(A.spill, spill_ints(blocks) * sizeof(int))) return fail(RTG_E_NOMEM, "spill");
+    if (A.ws.grow(A.ws.spill, spill_ints(blocks) * sizeof(int))) return fail(RTG_E_NOMEM, "spill");
     MegaArgs a;
     a.W = W, a.H = H, a.spp = p.spp > 0 ? p.spp : 1, a.max_level = p.max_level, a.tile_compat = p.tile_compat;
     a.bw = (W + 15) / 16;
     a.nblocks = a.bw * ((H + 7) / 8);
     a.dpf = (float)dp;
     a.seed = p.seed;
-    a.ax = A.tables.as<float>();
-    a.by = A.tables.as<float>() + W;
+    a.ax = A.ws.tables.as<float>();
+    a.by = A.ws.tables.as<float>() + W;
     a.rgb = o.rgb, a.ids = o.ids, a.tprim = o.tprim;
     a.counts = o.counts;
     a.cancel = A.cancel_dev;
-    a.spill = A.spill.as<int>();
+    a.spill = A.ws.spill.as<int>();
     const int kt = ktime_begin(A, stream, KT_MEGA, (uint64_t)W * H * a.spp);
     k_megakernel<<<blocks, kBlockThreads, 0, stream>>>(S.view, a);
     ktime_end(A, stream, kt);

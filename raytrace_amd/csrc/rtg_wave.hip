@@ -14,7 +14,20 @@
 //   resolve average of the samples of a pixel, quantised to RGB8
 // The per-node arithmetic is the megakernel's (shade_local, compose), so the
 // two engines give bit-identical frames.
+//
+// Batches.  The kernels are templated on B.  B = false renders one request
+// whose parameters and scene are kernel arguments.  B = true renders many
+// requests, over many scenes, as one wavefront: level 0 concatenates every
+// request's samples (each request's range is a whole number of 64-sample
+// chunks, so a wave of primary rays belongs to one request); deeper queues
+// mix requests, every ray and node carrying its request index.  A request's
+// rays see only its own scene, so its frame is identical to rendering it
+// alone.
+#include <algorithm>
+#include <cstring>
+
 #include "rt_devstack.hpp"
+#include "rt_prep.hpp"
 #include "rtg_internal.hpp"
 
 namespace rtg {
@@ -23,7 +36,7 @@ struct WRay {  // 48 B
     float ox, oy, oz, dx, dy, dz, bwc, mtlrfr;
     int link;  // 2 * parent node + slot (0 reflection, 1 refraction)
     int src_obj, src_prim;
-    int flags;  // bits 0-2 type, bit 3 inside, bits 4-6 source copy bin
+    int flags;  // bits 0-2 type, bit 3 inside, bits 4-6 source copy bin, bits 8-31 request
 };
 struct WHit {  // 32 B
     float t, w, u, v;
@@ -31,9 +44,12 @@ struct WHit {  // 32 B
 };
 struct WNode {  // 64 B
     float px, py, pz, nx, ny, nz, dx, dy, dz, vr, vg, vb, t;
-    int mtl, id_prim;
+    int mtl_req;  // bits 0-19 material, 20-31 request
+    int id_prim;
     int bits;  // bits 0-23 object, 24-26 copy bin, 27 valid, 28 inside, 29 total internal reflection
 };
+constexpr int kMtlBits = 20;
+constexpr int kMaxBatch = 1 << (32 - kMtlBits);
 
 // Device counters of one level, zeroed at the start of a frame; each in its
 // own 256-byte line.
@@ -44,40 +60,70 @@ struct LevelCtr {
 };
 constexpr int kCtrWords = (int)(sizeof(LevelCtr) / sizeof(unsigned));
 
-struct WaveArgs {
+// One request's render parameters: a kernel argument (B = false) or an entry
+// of the batch's request table (B = true).
+struct WaveReq {
     int W, H, spp, max_level, tile_compat;
-    int bw;         // 8x8 pixel blocks per row
-    int n_samples;  // pixel blocks * 64 * spp
+    int bw;           // 8x8 pixel blocks per row
+    int n_samples;    // pixel blocks * 64 * spp
+    int sample_base;  // first level-0 queue slot of the request
+    int scene;        // index into the batch's scene table
     float dpf;
     uint64_t seed;
-    const float *ax, *by;
+    const float *ax, *by;  // ray-generation tables (RayTracer.cpp:13-27)
+    uint8_t *rgb;
     int32_t *ids;
     float *tprim;
+};
+
+struct WaveArgs {
+    WaveReq r0;               // B = false
+    const WaveReq *reqs;      // B = true
+    const SceneView *scenes;  // B = true
+    const int *chunk_req;     // B = true: request of each 64-sample chunk of level 0 (-1: padding)
+    int n_samples;            // level-0 queue length
     unsigned long long *counts;
     int *spill;
 };
 
-__device__ inline bool sample_pixel(const WaveArgs &A, int s, int &px, int &py, int &si) {
-    const int per_block = 64 * A.spp;
+template <bool B>
+__device__ inline const WaveReq &req_at(const WaveArgs &A, int r) {
+    if constexpr (B)
+        return A.reqs[r];
+    else
+        return A.r0;
+}
+template <bool B>
+__device__ inline const SceneView &scene_at(const SceneView &S0, const WaveArgs &A, int sc) {
+    if constexpr (B)
+        return A.scenes[sc];
+    else
+        return S0;
+}
+
+__device__ inline bool sample_pixel(const WaveReq &R, int s, int &px, int &py, int &si) {
+    const int per_block = 64 * R.spp;
     const int pb = s / per_block, r = s - pb * per_block;
-    const int lane = r / A.spp;
-    si = r - lane * A.spp;
-    px = (pb % A.bw) * 8 + (lane & 7);
-    py = (pb / A.bw) * 8 + (lane >> 3);
-    if (px >= A.W || py >= A.H) return false;
-    if (A.tile_compat && (px >= ((A.W >> 6) << 6) || py >= ((A.H >> 6) << 6))) return false;
+    const int lane = r / R.spp;
+    si = r - lane * R.spp;
+    px = (pb % R.bw) * 8 + (lane & 7);
+    py = (pb / R.bw) * 8 + (lane >> 3);
+    if (px >= R.W || py >= R.H) return false;
+    if (R.tile_compat && (px >= ((R.W >> 6) << 6) || py >= ((R.H >> 6) << 6))) return false;
     return true;
 }
 
-__device__ inline TraceRay primary_ray_w(const SceneView &S, const WaveArgs &A, int px, int py, int s) {
-    float a = A.ax[px], b = A.by[py];
-    if (A.spp > 1) {
-        const uint64_t h = mix64(A.seed ^ ((uint64_t)(py * A.W + px) * 0x9E3779B97F4A7C15ull) ^
+// Primary ray of sample s of pixel (px, py): the reference's pixel-corner ray
+// for spp 1 (RayTracer.cpp:26-27), 2x2 jittered strata otherwise.
+__device__ inline TraceRay primary_ray_w(const SceneView &S, const WaveReq &R, int px, int py, int s) {
+    float a = R.ax[px], b = R.by[py];
+    if (R.spp > 1) {
+        const uint64_t h = mix64(R.seed ^ ((uint64_t)(py * R.W + px) * 0x9E3779B97F4A7C15ull) ^
                                  ((uint64_t)s * 0xC2B2AE3D27D4EB4Full));
         const float jx = (float)(h >> 40) * (1.0f / 16777216.0f);
         const float jy = (float)((h >> 16) & 0xFFFFFF) * (1.0f / 16777216.0f);
-        a = a + (((float)(s & 1) + jx) * 0.5f) * A.dpf;
-        b = b + (((float)((s >> 1) & 1) + jy) * 0.5f) * A.dpf;
+        a = a + (((float)(s & 1) + jx) * 0.5f) * R.dpf;
+        b = b + (((float)((s >> 1) & 1) + jy) * 0.5f) * R.dpf;
     }
     TraceRay q;
     q.r.o = S.cam.pos;
@@ -102,16 +148,20 @@ __device__ inline TraceRay load_ray(const WRay &w, float &bwc, float &mtlrfr) {
     return q;
 }
 
-__device__ inline WRay make_wray(const TraceRay &q, float bwc, float mtlrfr, int link) {
+__device__ inline WRay make_wray(const TraceRay &q, float bwc, float mtlrfr, int link, int req) {
     WRay w;
     w.ox = q.r.o.x, w.oy = q.r.o.y, w.oz = q.r.o.z;
     w.dx = q.r.d.x, w.dy = q.r.d.y, w.dz = q.r.d.z;
     w.bwc = bwc, w.mtlrfr = mtlrfr;
     w.link = link;
     w.src_obj = q.src.obj, w.src_prim = q.src.prim;
-    w.flags = (q.type & 7) | ((q.inside & 1) << 3) | ((q.src.bin & 7) << 4);
+    w.flags = (q.type & 7) | ((q.inside & 1) << 3) | ((q.src.bin & 7) << 4) | (req << 8);
     return w;
 }
+
+__device__ inline int node_req(const WNode &nd) { return (int)((unsigned)nd.mtl_req >> kMtlBits); }
+__device__ inline int node_mtl(const WNode &nd) { return nd.mtl_req & ((1 << kMtlBits) - 1); }
+__device__ inline SrcId node_id(const WNode &nd) { return SrcId{nd.bits & 0xFFFFFF, nd.id_prim, (nd.bits >> 24) & 7}; }
 
 __device__ inline void add_counts(unsigned long long *counts, uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
     a = wave_sum_u32(a), b = wave_sum_u32(b), c = wave_sum_u32(c), d = wave_sum_u32(d);
@@ -123,8 +173,23 @@ __device__ inline void add_counts(unsigned long long *counts, uint32_t a, uint32
     }
 }
 
-__global__ __launch_bounds__(kBlockThreads) void k_extend(SceneView S, WaveArgs A, int k, const WRay *rays,
-                                                          const unsigned *n_ptr, WHit *hits, LineCtr *sched) {
+__device__ inline uint32_t wave_or_u32(uint32_t v) {
+    for (int off = 32; off > 0; off >>= 1) v |= __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Request of level-0 sample i (-1: padding).
+template <bool B>
+__device__ inline int sample_req(const WaveArgs &A, int i) {
+    if constexpr (B)
+        return A.chunk_req[i >> 6];
+    else
+        return 0;
+}
+
+template <bool B>
+__global__ __launch_bounds__(kBlockThreads, 6) void k_extend(SceneView S0, WaveArgs A, int k, const WRay *rays,
+                                                             const unsigned *n_ptr, WHit *hits, LineCtr *sched) {
     __shared__ int lds[kLdsStack * kBlockThreads];
     DevStack st = make_stack(lds, A.spill);
     const int n = k == 0 ? A.n_samples : (int)*n_ptr;
@@ -133,15 +198,25 @@ __global__ __launch_bounds__(kBlockThreads) void k_extend(SceneView S, WaveArgs 
     for (int base = wq.next(); base >= 0; base = wq.next()) {
         const int i = base + __lane_id();
         if (i >= n) continue;
+        int r = 0;
+        WRay w{};
+        if (k == 0) {
+            r = sample_req<B>(A, i);
+        } else {
+            w = rays[i];
+            r = w.flags >> 8;
+        }
+        const WaveReq &R = req_at<B>(A, r < 0 ? 0 : r);
+        const SceneView &S = scene_at<B>(S0, A, R.scene);
         TraceRay q;
         int px = 0, py = 0, si = 0;
-        bool valid = true;
+        bool valid = r >= 0;
         if (k == 0) {
-            valid = sample_pixel(A, i, px, py, si);
-            if (valid) q = primary_ray_w(S, A, px, py, si);
+            if (valid) valid = sample_pixel(R, i - R.sample_base, px, py, si);
+            if (valid) q = primary_ray_w(S, R, px, py, si);
         } else {
             float bwc, mtlrfr;
-            q = load_ray(rays[i], bwc, mtlrfr);
+            q = load_ray(w, bwc, mtlrfr);
             if (q.src.obj >= 0) q.self_test = self_testable(S, q.r, q.src, kMiss);
         }
         Hit h{kMiss, HIT_NONE, -1, -1, 0, 0, 0};
@@ -156,23 +231,23 @@ __global__ __launch_bounds__(kBlockThreads) void k_extend(SceneView S, WaveArgs 
         }
         hits[i] = WHit{h.t, h.w, h.u, h.v, h.kind, h.obj, h.prim, 0};
         if (k == 0 && valid && si == 0) {
-            const size_t pix = (size_t)py * A.W + px;
-            if (A.ids) {
-                A.ids[2 * pix] = h.obj;
-                A.ids[2 * pix + 1] = id_prim(S, h);
+            const size_t pix = (size_t)py * R.W + px;
+            if (R.ids) {
+                R.ids[2 * pix] = h.obj;
+                R.ids[2 * pix + 1] = id_prim(S, h);
             }
-            if (A.tprim) A.tprim[pix] = h.t;
+            if (R.tprim) R.tprim[pix] = h.t;
         }
     }
     add_counts(A.counts, c_prim, 0, c_refl, c_refr);
 }
 
-__global__ __launch_bounds__(kBlockThreads) void k_shade(SceneView S, WaveArgs A, int k, const WRay *rays,
+template <bool B>
+__global__ __launch_bounds__(kBlockThreads) void k_shade(SceneView S0, WaveArgs A, int k, const WRay *rays,
                                                          const unsigned *n_ptr, const WHit *hits, WNode *nodes,
-                                                         unsigned *masks, WRay *next, LevelCtr *ctr, unsigned *shadow,
-                                                         uint32_t lights_on) {
+                                                         unsigned *masks, WRay *next, LevelCtr *ctr,
+                                                         unsigned *shadow) {
     const int n = k == 0 ? A.n_samples : (int)*n_ptr;
-    const int n_on = __builtin_popcount(lights_on);
     WaveQueue wq(ctr->sched[1], n);
     uint32_t c_shadow = 0;
     for (int base = wq.next(); base >= 0; base = wq.next()) {
@@ -181,16 +256,29 @@ __global__ __launch_bounds__(kBlockThreads) void k_shade(SceneView S, WaveArgs A
         bool lit = false;
         WRay c0, c1;  // children in append order (registers, no indexed array)
         V3 P = V3{0, 0, 0}, N = V3{0, 0, 0}, D = V3{0, 0, 0};
+        uint32_t on = 0;  // switched-on lights of the ray's scene
+        const SceneView *Sp = &S0;
         if (i < n) {
+            int r = 0;
+            WRay w{};
+            if (k == 0) {
+                r = sample_req<B>(A, i);
+            } else {
+                w = rays[i];
+                r = w.flags >> 8;
+            }
+            const WaveReq &R = req_at<B>(A, r < 0 ? 0 : r);
+            const SceneView &S = scene_at<B>(S0, A, R.scene);
+            Sp = &S;
             TraceRay q;
             float bwc = 1.0f, mtlrfr = 1.0f;
-            bool valid = true;
+            bool valid = r >= 0;
             if (k == 0) {
                 int px, py, si;
-                valid = sample_pixel(A, i, px, py, si);
-                if (valid) q = primary_ray_w(S, A, px, py, si);
+                if (valid) valid = sample_pixel(R, i - R.sample_base, px, py, si);
+                if (valid) q = primary_ray_w(S, R, px, py, si);
             } else {
-                q = load_ray(rays[i], bwc, mtlrfr);
+                q = load_ray(w, bwc, mtlrfr);
             }
             const WHit wh = hits[i];
             const float zNear = k == 0 ? S.cam.zNear : 0.0f;
@@ -202,20 +290,20 @@ __global__ __launch_bounds__(kBlockThreads) void k_shade(SceneView S, WaveArgs A
                 const V3 vc = texel(S, hr.tex, hr.tu, hr.tv);
                 const DevMaterial &m = S.mtls[hr.mtl];
                 int tir = 0;
-                if (m.reflect > 0.01f && k + 1 <= A.max_level && bwc * m.reflect >= 1e-5f) {
-                    c0 = make_wray(reflect_ray(q, hr), bwc * m.reflect, 1.0f, 2 * i);
+                if (m.reflect > 0.01f && k + 1 <= R.max_level && bwc * m.reflect >= 1e-5f) {
+                    c0 = make_wray(reflect_ray(q, hr), bwc * m.reflect, 1.0f, 2 * i, r);
                     n_child = 1;
                 }
                 if (m.refract > 0.01f) {
                     TraceRay cq;
                     if (!refract_ray(q, mtlrfr, hr, cq)) {
                         tir = 1;
-                    } else if (k + 1 <= A.max_level && bwc * m.refract >= 1e-5f) {
-                        const WRay w = make_wray(cq, bwc * m.refract, hr.rfr, 2 * i + 1);
+                    } else if (k + 1 <= R.max_level && bwc * m.refract >= 1e-5f) {
+                        const WRay cw = make_wray(cq, bwc * m.refract, hr.rfr, 2 * i + 1, r);
                         if (n_child == 0)
-                            c0 = w;
+                            c0 = cw;
                         else
-                            c1 = w;
+                            c1 = cw;
                         ++n_child;
                     }
                 }
@@ -224,13 +312,14 @@ __global__ __launch_bounds__(kBlockThreads) void k_shade(SceneView S, WaveArgs A
                 nd.dx = q.r.d.x, nd.dy = q.r.d.y, nd.dz = q.r.d.z;
                 nd.vr = vc.x, nd.vg = vc.y, nd.vb = vc.z;
                 nd.t = hr.t;
-                nd.mtl = hr.mtl;
+                nd.mtl_req = hr.mtl | (r << kMtlBits);
                 nd.id_prim = hr.id.prim;
                 nd.bits = (hr.id.obj & 0xFFFFFF) | ((hr.id.bin & 7) << 24) | (1 << 27) | ((hr.inside & 1) << 28) |
                           (tir << 29);
                 lit = true;
                 P = hr.P, N = hr.N, D = q.r.d;
-                c_shadow += n_on;  // the reference traces one shadow ray per switched-on light
+                on = S.lights_on;
+                c_shadow += __builtin_popcount(on);  // the reference traces one shadow ray per switched-on light
             }
             nodes[i] = nd;
             masks[i] = 0;
@@ -243,30 +332,30 @@ __global__ __launch_bounds__(kBlockThreads) void k_shade(SceneView S, WaveArgs A
         // one light.  Lights that cannot change the colour are not traced
         // (shade_local ignores their mask bit).
         uint32_t emit = 0;
-        if (lit) {
-            int r = 0;
-            for (uint32_t lm = lights_on; lm; lm &= lm - 1, ++r)
-                if (light_dots(N, D, light_term(S.lights[__builtin_ctz(lm)], P)).matters()) emit |= 1u << r;
-        }
+        if (lit)
+            for (uint32_t lm = on; lm; lm &= lm - 1) {
+                const int li = __builtin_ctz(lm);
+                if (light_dots(N, D, light_term(Sp->lights[li], P)).matters()) emit |= 1u << li;
+            }
+        const uint32_t any = B ? wave_or_u32(emit) : S0.lights_on;
         int total = 0;
-        for (int r = 0; r < n_on; ++r) total += __popcll(__ballot((emit >> r) & 1u));
+        for (uint32_t lm = any; lm; lm &= lm - 1) total += __popcll(__ballot((emit >> __builtin_ctz(lm)) & 1u));
         int sb = 0;
         if (__lane_id() == 0 && total > 0) sb = (int)atomicAdd(&ctr->n_shadow.v, (unsigned)total);
         sb = __shfl(sb, 0, 64);
         const unsigned long long below = (1ull << __lane_id()) - 1ull;
-        int r = 0;
-        for (uint32_t lm = lights_on; lm; lm &= lm - 1, ++r) {
-            const unsigned long long m = __ballot((emit >> r) & 1u);
-            if ((emit >> r) & 1u) shadow[sb + __popcll(m & below)] = ((unsigned)i << 5) | (unsigned)__builtin_ctz(lm);
+        for (uint32_t lm = any; lm; lm &= lm - 1) {
+            const int li = __builtin_ctz(lm);
+            const unsigned long long m = __ballot((emit >> li) & 1u);
+            if ((emit >> li) & 1u) shadow[sb + __popcll(m & below)] = ((unsigned)i << 5) | (unsigned)li;
             sb += __popcll(m);
         }
     }
     add_counts(A.counts, 0, c_shadow, 0, 0);
 }
 
-__device__ inline SrcId node_id(const WNode &nd) { return SrcId{nd.bits & 0xFFFFFF, nd.id_prim, (nd.bits >> 24) & 7}; }
-
-__global__ __launch_bounds__(kBlockThreads) void k_occlude(SceneView S, WaveArgs A, const WNode *nodes,
+template <bool B>
+__global__ __launch_bounds__(kBlockThreads) void k_occlude(SceneView S0, WaveArgs A, const WNode *nodes,
                                                            const unsigned *shadow, LevelCtr *ctr, unsigned *masks) {
     __shared__ int lds[kLdsStack * kBlockThreads];
     DevStack st = make_stack(lds, A.spill);
@@ -278,6 +367,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_occlude(SceneView S, WaveArgs
         const unsigned e = shadow[j];
         const int i = (int)(e >> 5), li = (int)(e & 31);
         const WNode nd = nodes[i];
+        const SceneView &S = scene_at<B>(S0, A, req_at<B>(A, node_req(nd)).scene);
         const V3 P = V3{nd.px, nd.py, nd.pz};
         const LightTerm lt = light_term(S.lights[li], P);
         TraceRay sr;
@@ -297,7 +387,8 @@ __global__ void k_fill_slots(float4 *slots, const unsigned *n_ptr, int n_fixed) 
         slots[i] = make_float4(0.0f, 0.0f, 0.0f, kMiss);
 }
 
-__global__ __launch_bounds__(kBlockThreads) void k_combine(SceneView S, WaveArgs A, int k, const WRay *rays,
+template <bool B>
+__global__ __launch_bounds__(kBlockThreads) void k_combine(SceneView S0, WaveArgs A, int k, const WRay *rays,
                                                            const unsigned *n_ptr, const WNode *nodes,
                                                            const unsigned *masks, const float4 *slots,
                                                            float4 *parent_slots, float4 *samples) {
@@ -309,11 +400,13 @@ __global__ __launch_bounds__(kBlockThreads) void k_combine(SceneView S, WaveArgs
         const WNode nd = nodes[i];
         float4 out = make_float4(0.0f, 0.0f, 0.0f, kMiss);
         if ((nd.bits >> 27) & 1) {
+            const SceneView &S = scene_at<B>(S0, A, req_at<B>(A, node_req(nd)).scene);
+            const int mtl = node_mtl(nd);
             const unsigned mask = masks[i];
             const V3 L = shade_local(S, V3{nd.dx, nd.dy, nd.dz}, V3{nd.px, nd.py, nd.pz}, V3{nd.nx, nd.ny, nd.nz},
-                                     nd.mtl, V3{nd.vr, nd.vg, nd.vb},
+                                     mtl, V3{nd.vr, nd.vg, nd.vb},
                                      [&](int li, const LightTerm &, bool) -> bool { return (mask >> li) & 1u; });
-            const DevMaterial &m = S.mtls[nd.mtl];
+            const DevMaterial &m = S.mtls[mtl];
             const float4 a = slots[2 * i], b = slots[2 * i + 1];
             const V3 col = compose(L, m.reflect, m.refract, (nd.bits >> 29) & 1, (nd.bits >> 28) & 1, m.kd,
                                    V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, b.w);
@@ -326,22 +419,26 @@ __global__ __launch_bounds__(kBlockThreads) void k_combine(SceneView S, WaveArgs
     }
 }
 
-__global__ void k_resolve(WaveArgs A, const float4 *samples, uint8_t *rgb) {
-    const int npix = A.W * A.H;
+// Box filter of a request's samples, quantised (RayTracer.cpp:34-37,
+// 3DElement.cpp:463-468).  Grid y = request.
+template <bool B>
+__global__ void k_resolve(WaveArgs A, const float4 *samples) {
+    const WaveReq &R = req_at<B>(A, blockIdx.y);
+    const int npix = R.W * R.H;
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += gridDim.x * blockDim.x) {
-        const int px = p % A.W, py = p / A.W;
-        if (A.tile_compat && (px >= ((A.W >> 6) << 6) || py >= ((A.H >> 6) << 6))) continue;
-        const int pb = (py >> 3) * A.bw + (px >> 3), lane = ((py & 7) << 3) | (px & 7);
-        const int s0 = (pb * 64 + lane) * A.spp;
+        const int px = p % R.W, py = p / R.W;
+        if (R.tile_compat && (px >= ((R.W >> 6) << 6) || py >= ((R.H >> 6) << 6))) continue;
+        const int pb = (py >> 3) * R.bw + (px >> 3), lane = ((py & 7) << 3) | (px & 7);
+        const int s0 = R.sample_base + (pb * 64 + lane) * R.spp;
         V3 acc = V3{0, 0, 0};
-        for (int s = 0; s < A.spp; ++s) {
+        for (int s = 0; s < R.spp; ++s) {
             const float4 c = samples[s0 + s];
             acc = acc + V3{c.x, c.y, c.z};
         }
-        if (A.spp > 1) acc = acc * (1.0f / (float)A.spp);
-        rgb[3 * (size_t)p + 0] = quantize(acc.x);
-        rgb[3 * (size_t)p + 1] = quantize(acc.y);
-        rgb[3 * (size_t)p + 2] = quantize(acc.z);
+        if (R.spp > 1) acc = acc * (1.0f / (float)R.spp);
+        R.rgb[3 * (size_t)p + 0] = quantize(acc.x);
+        R.rgb[3 * (size_t)p + 1] = quantize(acc.y);
+        R.rgb[3 * (size_t)p + 2] = quantize(acc.z);
     }
 }
 
@@ -350,12 +447,32 @@ __global__ void k_zero_u32(unsigned *p, int n) {
     if (i < n) p[i] = 0;
 }
 
+// Batch: request of every 64-sample chunk of level 0 (binary search over the
+// requests' sample ranges; -1 for padding chunks).
+__global__ void k_chunk_req(const WaveReq *reqs, int n_reqs, int *chunk_req, int n_chunks) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n_chunks) return;
+    const int s = c << 6;
+    int lo = 0, hi = n_reqs - 1, r = -1;
+    while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        if (reqs[mid].sample_base <= s) {
+            r = mid;
+            lo = mid + 1;
+        } else {
+            hi = mid - 1;
+        }
+    }
+    if (r >= 0 && s >= reqs[r].sample_base + reqs[r].n_samples) r = -1;
+    chunk_req[c] = r;
+}
+
 // Largest queue memory (bytes) a frame may reserve to run without host round
 // trips between levels.
 constexpr size_t kNoSyncBudget = size_t(8) << 30;
 
 // Resident blocks per CU of a kernel at kBlockThreads threads (registers and
-// LDS permitting), queried once.
+// LDS permitting).
 template <class K>
 static int blocks_per_cu(K kernel) {
     int nb = 0;
@@ -363,109 +480,321 @@ static int blocks_per_cu(K kernel) {
     return nb;
 }
 
-int launch_wavefront(const SceneRes &S, const rtg_params &p, int W, int H, const RenderOut &o, hipStream_t stream,
-                     Arena &A) {
-    double dp;
-    int st = upload_raygen_tables(S, W, H, stream, A, dp);
-    if (st != RTG_OK) return st;
-    const int cus = A.cus > 0 ? A.cus : 256;
-    static const int occ_extend = blocks_per_cu(k_extend), occ_shade = blocks_per_cu(k_shade),
-                     occ_occlude = blocks_per_cu(k_occlude);
-    const int g_extend = cus * occ_extend, g_shade = cus * occ_shade, g_occlude = cus * occ_occlude;
-    if (ensure(A.spill, spill_ints(std::max(g_extend, g_occlude)) * sizeof(int))) return fail(RTG_E_NOMEM, "spill");
-    constexpr size_t kCtrBytes = (size_t)Arena::kLevels * sizeof(LevelCtr);
-    if (A.qcount_host == nullptr)
-        RTG_HIP(hipHostMalloc(reinterpret_cast<void **>(&A.qcount_host), kCtrBytes, hipHostMallocDefault));
-    if (ensure(A.w_qcount, kCtrBytes)) return fail(RTG_E_NOMEM, "queue counters");
-    WaveArgs a;
-    a.W = W, a.H = H, a.spp = p.spp > 0 ? p.spp : 1, a.max_level = p.max_level, a.tile_compat = p.tile_compat;
-    a.bw = (W + 7) / 8;
-    a.n_samples = a.bw * ((H + 7) / 8) * 64 * a.spp;
-    a.dpf = (float)dp;
-    a.seed = p.seed;
-    a.ax = A.tables.as<float>();
-    a.by = A.tables.as<float>() + W;
-    a.ids = o.ids, a.tprim = o.tprim;
-    a.counts = o.counts;
-    a.spill = A.spill.as<int>();
-    const uint32_t lights_on = S.lights_on;  // lights that trace shadow rays (Light::bLight)
-    const int n_on = __builtin_popcount(lights_on);
-    LevelCtr *ctr = A.w_qcount.as<LevelCtr>();
-    const int ctr_words = Arena::kLevels * kCtrWords;
-    k_zero_u32<<<(ctr_words + 255) / 256, 256, 0, stream>>>(A.w_qcount.as<unsigned>(), ctr_words);
-    auto grid = [](int g, size_t items) { return (int)std::max<size_t>(8, std::min<size_t>(g, (items + 127) / 128)); };
-    // Queue capacity per level: level k holds at most n_samples * fanout^k
-    // rays.  When those bounds fit the budget, every queue is sized by its
-    // bound and the frame is enqueued without a host round trip (kernels read
-    // the live counts from the device); otherwise each level is sized from
-    // the previous level's count, read back to the host.
+template <bool B>
+struct Grids {
+    int extend, shade, occlude;
+    explicit Grids(int cus) {
+        static const int oe = blocks_per_cu(k_extend<B>), os = blocks_per_cu(k_shade<B>),
+                         oo = blocks_per_cu(k_occlude<B>);
+        extend = cus * oe, shade = cus * os, occlude = cus * oo;
+    }
+};
+
+// Shape of a wavefront: what bounds its queues.
+struct WavePlan {
+    int n_samples;  // level-0 queue length
+    int max_level;  // deepest level any request traces
+    int fanout;     // most child rays per hit
+    int n_on;       // most switched-on lights of a scene
+    int n_reqs;     // grid y of k_resolve
+    size_t bound[Workspace::kLevels];
+    bool no_sync;
+};
+
+static WavePlan make_plan(int n_samples, int max_level, int fanout, int n_on, int n_reqs) {
+    WavePlan pl{};
+    pl.n_samples = n_samples, pl.max_level = max_level, pl.fanout = fanout, pl.n_on = n_on, pl.n_reqs = n_reqs;
     const size_t per_ray = sizeof(WRay) + sizeof(WHit) + sizeof(WNode) + sizeof(unsigned) + 2 * sizeof(float4) +
                            (size_t)n_on * sizeof(unsigned);
-    size_t bound[Arena::kLevels] = {};
     size_t need = 0;
-    bool no_sync = true;
-    for (int k = 0; k <= p.max_level; ++k) {
-        bound[k] = k == 0 ? (size_t)a.n_samples : bound[k - 1] * (size_t)S.max_fanout;
-        need += bound[k] * per_ray;
-        if (need > kNoSyncBudget) no_sync = false;
+    pl.no_sync = true;
+    for (int k = 0; k <= max_level; ++k) {
+        pl.bound[k] = k == 0 ? (size_t)n_samples : std::min<size_t>(pl.bound[k - 1] * (size_t)fanout, size_t(1) << 40);
+        need += pl.bound[k] * per_ray;
+        if (need > kNoSyncBudget) pl.no_sync = false;
     }
-    size_t n_k = (size_t)a.n_samples;
+    return pl;
+}
+
+// Grow the level-k buffers for n_k rays.
+static int reserve_level(Workspace &ws, const WavePlan &pl, int k, size_t n_k) {
+    const size_t cap_next = k < pl.max_level ? (size_t)std::max(1, pl.fanout) * n_k : 1;
+    const int kn = k + 1 < Workspace::kLevels ? k + 1 : k;
+    if (ws.grow(ws.w_hits[k], n_k * sizeof(WHit)) || ws.grow(ws.w_nodes[k], n_k * sizeof(WNode)) ||
+        ws.grow(ws.w_masks[k], n_k * sizeof(unsigned)) || ws.grow(ws.w_slots[k], 2 * n_k * sizeof(float4)) ||
+        ws.grow(ws.w_shadow[k], std::max<size_t>(1, n_k * pl.n_on) * sizeof(unsigned)) ||
+        ws.grow(ws.w_rays[kn], cap_next * sizeof(WRay)))
+        return fail(RTG_E_NOMEM, "wavefront queues");
+    return RTG_OK;
+}
+
+// Buffers of a no-sync plan, all sized up front (a captured graph must not
+// see them move).
+static int reserve_all(Workspace &ws, const WavePlan &pl, int cus) {
+    const Grids<false> g0(cus);
+    const Grids<true> g1(cus);
+    const int gmax = std::max(std::max(g0.extend, g0.occlude), std::max(g1.extend, g1.occlude));
+    if (ws.grow(ws.spill, spill_ints(gmax) * sizeof(int))) return fail(RTG_E_NOMEM, "spill");
+    if (ws.grow(ws.w_qcount, (size_t)Workspace::kLevels * sizeof(LevelCtr))) return fail(RTG_E_NOMEM, "counters");
+    if (ws.grow(ws.w_samples, (size_t)pl.n_samples * sizeof(float4))) return fail(RTG_E_NOMEM, "samples");
+    for (int k = 0; k <= pl.max_level && pl.bound[k] > 0; ++k) {
+        const int st = reserve_level(ws, pl, k, pl.bound[k]);
+        if (st != RTG_OK) return st;
+    }
+    return RTG_OK;
+}
+
+// Enqueue the levels of a wavefront on `stream`.  With pl.no_sync every
+// buffer must already hold its bound (reserve_all) and nothing here
+// allocates or waits.
+template <bool B>
+static int enqueue_wave(const SceneView &S0, const WaveArgs &a, const WavePlan &pl, hipStream_t stream,
+                        Workspace &ws, Arena &A, const int *cancel_host) {
+    const int cus = A.cus > 0 ? A.cus : 256;
+    const Grids<B> g(cus);
+    if (!pl.no_sync) {
+        if (ws.grow(ws.spill, spill_ints(std::max(g.extend, g.occlude)) * sizeof(int)))
+            return fail(RTG_E_NOMEM, "spill");
+        if (ws.grow(ws.w_qcount, (size_t)Workspace::kLevels * sizeof(LevelCtr)))
+            return fail(RTG_E_NOMEM, "queue counters");
+    }
+    LevelCtr *ctr = ws.w_qcount.as<LevelCtr>();
+    const int ctr_words = Workspace::kLevels * kCtrWords;
+    k_zero_u32<<<(ctr_words + 255) / 256, 256, 0, stream>>>(ws.w_qcount.as<unsigned>(), ctr_words);
+    auto grid = [](int gmax, size_t items) {
+        return (int)std::max<size_t>(8, std::min<size_t>(gmax, (items + 127) / 128));
+    };
+    WaveArgs args = a;
+    args.spill = ws.spill.as<int>();
+    size_t n_k = (size_t)pl.n_samples;
     int last = -1;
-    for (int k = 0; k <= p.max_level && n_k > 0; ++k) {
-        if (*A.cancel_host) return RTG_OK;
-        const size_t cap_next = k < p.max_level ? (size_t)std::max(1, S.max_fanout) * n_k : 1;
-        if (ensure(A.w_hits[k], n_k * sizeof(WHit)) || ensure(A.w_nodes[k], n_k * sizeof(WNode)) ||
-            ensure(A.w_masks[k], n_k * sizeof(unsigned)) || ensure(A.w_slots[k], 2 * n_k * sizeof(float4)) ||
-            ensure(A.w_shadow[k], std::max<size_t>(1, n_k * n_on) * sizeof(unsigned)) ||
-            ensure(A.w_rays[k + 1 < Arena::kLevels ? k + 1 : k], cap_next * sizeof(WRay)))
-            return fail(RTG_E_NOMEM, "wavefront queues");
-        const WRay *rays = k == 0 ? nullptr : A.w_rays[k].as<WRay>();
+    for (int k = 0; k <= pl.max_level && n_k > 0; ++k) {
+        if (cancel_host && *cancel_host) return RTG_OK;
+        if (!pl.no_sync) {
+            const int st = reserve_level(ws, pl, k, n_k);
+            if (st != RTG_OK) return st;
+        }
+        const WRay *rays = k == 0 ? nullptr : ws.w_rays[k].as<WRay>();
         const unsigned *n_ptr = k == 0 ? nullptr : &ctr[k - 1].n_next.v;
-        WRay *next = A.w_rays[k + 1 < Arena::kLevels ? k + 1 : k].as<WRay>();
+        WRay *next = ws.w_rays[k + 1 < Workspace::kLevels ? k + 1 : k].as<WRay>();
         int kt = ktime_begin(A, stream, KT_EXTEND, n_k);
-        k_extend<<<grid(g_extend, n_k), kBlockThreads, 0, stream>>>(S.view, a, k, rays, n_ptr,
-                                                                    A.w_hits[k].as<WHit>(), ctr[k].sched[0]);
-        ktime_end(A, stream, kt);
+        k_extend<B><<<grid(g.extend, n_k), kBlockThreads, 0, stream>>>(S0, args, k, rays, n_ptr,
+                                                                       ws.w_hits[k].as<WHit>(), ctr[k].sched[0]);
+        ktime_end(A, stream, kt, n_ptr);
         kt = ktime_begin(A, stream, KT_SHADE, n_k);
-        k_shade<<<grid(g_shade, n_k), kBlockThreads, 0, stream>>>(
-            S.view, a, k, rays, n_ptr, A.w_hits[k].as<WHit>(), A.w_nodes[k].as<WNode>(), A.w_masks[k].as<unsigned>(),
-            next, ctr + k, A.w_shadow[k].as<unsigned>(), lights_on);
-        ktime_end(A, stream, kt);
-        k_fill_slots<<<256, 256, 0, stream>>>(A.w_slots[k].as<float4>(), n_ptr, k == 0 ? a.n_samples : -1);
-        if (n_on > 0) {
+        k_shade<B><<<grid(g.shade, n_k), kBlockThreads, 0, stream>>>(
+            S0, args, k, rays, n_ptr, ws.w_hits[k].as<WHit>(), ws.w_nodes[k].as<WNode>(),
+            ws.w_masks[k].as<unsigned>(), next, ctr + k, ws.w_shadow[k].as<unsigned>());
+        ktime_end(A, stream, kt, n_ptr);
+        k_fill_slots<<<256, 256, 0, stream>>>(ws.w_slots[k].as<float4>(), n_ptr, k == 0 ? pl.n_samples : -1);
+        if (pl.n_on > 0) {
             kt = ktime_begin(A, stream, KT_OCCLUDE, 0);
-            k_occlude<<<grid(g_occlude, n_k * n_on), kBlockThreads, 0, stream>>>(
-                S.view, a, A.w_nodes[k].as<WNode>(), A.w_shadow[k].as<unsigned>(), ctr + k,
-                A.w_masks[k].as<unsigned>());
+            k_occlude<B><<<grid(g.occlude, n_k * pl.n_on), kBlockThreads, 0, stream>>>(
+                S0, args, ws.w_nodes[k].as<WNode>(), ws.w_shadow[k].as<unsigned>(), ctr + k,
+                ws.w_masks[k].as<unsigned>());
             ktime_end(A, stream, kt, &ctr[k].n_shadow.v);
         }
         RTG_HIP(hipGetLastError());
         last = k;
-        if (k == p.max_level) break;
-        if (no_sync) {
-            n_k = bound[k + 1];
+        if (k == pl.max_level) break;
+        if (pl.no_sync) {
+            n_k = pl.bound[k + 1];
             continue;
         }
-        RTG_HIP(hipMemcpyAsync(A.qcount_host, &ctr[k].n_next.v, sizeof(unsigned), hipMemcpyDeviceToHost, stream));
+        RTG_HIP(hipMemcpyAsync(ws.qcount_host, &ctr[k].n_next.v, sizeof(unsigned), hipMemcpyDeviceToHost, stream));
         RTG_HIP(hipStreamSynchronize(stream));
-        n_k = A.qcount_host[0];
+        n_k = ws.qcount_host[0];
     }
-    if (*A.cancel_host) return RTG_OK;
-    if (ensure(A.w_samples, (size_t)a.n_samples * sizeof(float4))) return fail(RTG_E_NOMEM, "samples");
+    if (cancel_host && *cancel_host) return RTG_OK;
+    if (!pl.no_sync && ws.grow(ws.w_samples, (size_t)pl.n_samples * sizeof(float4)))
+        return fail(RTG_E_NOMEM, "samples");
     const int blocks = std::max(8, cus * 8);
     for (int k = last; k >= 0; --k) {
-        const WRay *rays = k == 0 ? nullptr : A.w_rays[k].as<WRay>();
+        const WRay *rays = k == 0 ? nullptr : ws.w_rays[k].as<WRay>();
         const unsigned *n_ptr = k == 0 ? nullptr : &ctr[k - 1].n_next.v;
-        const int kt = ktime_begin(A, stream, KT_COMBINE, k == 0 ? (uint64_t)a.n_samples : 0);
-        k_combine<<<blocks, kBlockThreads, 0, stream>>>(
-            S.view, a, k, rays, n_ptr, A.w_nodes[k].as<WNode>(), A.w_masks[k].as<unsigned>(),
-            A.w_slots[k].as<float4>(), k > 0 ? A.w_slots[k - 1].as<float4>() : nullptr, A.w_samples.as<float4>());
+        const int kt = ktime_begin(A, stream, KT_COMBINE, k == 0 ? (uint64_t)pl.n_samples : 0);
+        k_combine<B><<<blocks, kBlockThreads, 0, stream>>>(
+            S0, args, k, rays, n_ptr, ws.w_nodes[k].as<WNode>(), ws.w_masks[k].as<unsigned>(),
+            ws.w_slots[k].as<float4>(), k > 0 ? ws.w_slots[k - 1].as<float4>() : nullptr,
+            ws.w_samples.as<float4>());
         ktime_end(A, stream, kt, n_ptr);
     }
-    const int kt = ktime_begin(A, stream, KT_RESOLVE, (uint64_t)W * H);
-    k_resolve<<<1024, 256, 0, stream>>>(a, A.w_samples.as<float4>(), o.rgb);
+    const int kt = ktime_begin(A, stream, KT_RESOLVE, 0);
+    k_resolve<B><<<dim3(B ? 256 : 1024, pl.n_reqs), 256, 0, stream>>>(args, ws.w_samples.as<float4>());
     ktime_end(A, stream, kt);
     RTG_HIP(hipGetLastError());
+    return RTG_OK;
+}
+
+static int host_buffers(Workspace &ws) {
+    if (ws.qcount_host == nullptr)
+        RTG_HIP(hipHostMalloc(reinterpret_cast<void **>(&ws.qcount_host), sizeof(unsigned) * 4, hipHostMallocDefault));
+    return RTG_OK;
+}
+
+int launch_wavefront(const SceneRes &S, const rtg_params &p, int W, int H, const RenderOut &o, hipStream_t stream,
+                     Arena &A) {
+    Workspace &ws = A.ws;
+    double dp;
+    int st = upload_raygen_tables(S, W, H, stream, ws, dp);
+    if (st != RTG_OK) return st;
+    if ((st = host_buffers(ws)) != RTG_OK) return st;
+    WaveArgs a{};
+    WaveReq &r = a.r0;
+    r.W = W, r.H = H, r.spp = p.spp > 0 ? p.spp : 1, r.max_level = p.max_level, r.tile_compat = p.tile_compat;
+    r.bw = (W + 7) / 8;
+    r.n_samples = r.bw * ((H + 7) / 8) * 64 * r.spp;
+    r.sample_base = 0;
+    r.scene = 0;
+    r.dpf = (float)dp;
+    r.seed = p.seed;
+    r.ax = ws.tables.as<float>();
+    r.by = ws.tables.as<float>() + W;
+    r.rgb = o.rgb, r.ids = o.ids, r.tprim = o.tprim;
+    a.n_samples = r.n_samples;
+    a.counts = o.counts;
+    WavePlan pl = make_plan(r.n_samples, p.max_level, S.max_fanout, __builtin_popcount(S.lights_on), 1);
+    if (pl.no_sync && (st = reserve_all(ws, pl, A.cus > 0 ? A.cus : 256)) != RTG_OK) return st;
+    return enqueue_wave<false>(S.view, a, pl, stream, ws, A, A.cancel_host);
+}
+
+// Pinned staging for batch tables; replaced only while `stream` is idle.
+static int staging(Workspace &ws, size_t bytes, hipStream_t stream) {
+    if (ws.staging_bytes >= bytes) return RTG_OK;
+    RTG_HIP(hipStreamSynchronize(stream));
+    if (ws.staging) (void)hipHostFree(ws.staging);
+    ws.staging = nullptr;
+    ws.staging_bytes = 0;
+    const size_t n = std::max<size_t>(bytes, 1 << 16) * 5 / 4;
+    RTG_HIP(hipHostMalloc(&ws.staging, n, hipHostMallocDefault));
+    ws.staging_bytes = n;
+    return RTG_OK;
+}
+
+int launch_batch(const BatchItem *items, int n, unsigned long long *counts, hipStream_t stream, Workspace &ws,
+                 Arena &A, bool use_graph, int *graph_replayed) {
+    if (graph_replayed) *graph_replayed = 0;
+    if (n <= 0) return RTG_OK;
+    if (n > kMaxBatch) return fail(RTG_E_LIMIT, "batch larger than 4096 requests");
+    // distinct scenes, request table, ray-generation tables
+    std::vector<const SceneRes *> scenes;
+    std::vector<int> scene_of(n);
+    int fanout = 0, n_on = 0, max_level = 0;
+    size_t table_floats = 0;
+    for (int i = 0; i < n; ++i) {
+        const SceneRes *s = items[i].scene;
+        auto it = std::find(scenes.begin(), scenes.end(), s);
+        scene_of[i] = (int)(it - scenes.begin());
+        if (it == scenes.end()) scenes.push_back(s);
+        fanout = std::max(fanout, s->max_fanout);
+        n_on = std::max(n_on, __builtin_popcount(s->lights_on));
+        max_level = std::max(max_level, (int)items[i].params.max_level);
+        table_floats += (size_t)items[i].W + items[i].H;
+    }
+    // Pad the request count and the level-0 length to buckets so that
+    // batches of similar size share one captured graph.
+    int n_pad = 1;
+    while (n_pad < n) n_pad <<= 1;
+    std::vector<WaveReq> reqs(n_pad);
+    size_t total = 0;
+    for (int i = 0; i < n; ++i) {
+        const BatchItem &it = items[i];
+        WaveReq &r = reqs[i];
+        r.W = it.W, r.H = it.H, r.spp = it.params.spp > 0 ? it.params.spp : 1;
+        r.max_level = it.params.max_level, r.tile_compat = it.params.tile_compat;
+        r.bw = (it.W + 7) / 8;
+        r.n_samples = r.bw * ((it.H + 7) / 8) * 64 * r.spp;
+        r.sample_base = (int)total;
+        r.scene = scene_of[i];
+        r.seed = it.params.seed;
+        r.rgb = it.rgb, r.ids = it.ids, r.tprim = it.tprim;
+        total += (size_t)r.n_samples;
+    }
+    if (total > (size_t)INT32_MAX / 2) return fail(RTG_E_LIMIT, "batch has too many samples");
+    const size_t padded = batch_bucket(total);
+    // Level 0 is read in whole 64-sample chunks (chunk_req holds one entry per
+    // chunk): a length that is not a multiple of 64 would read past it.
+    if (padded % 64 != 0 || padded < total) return fail(RTG_E_STATE, "batch bucket is not whole chunks");
+    for (int i = n; i < n_pad; ++i) {
+        WaveReq &r = reqs[i];
+        r = WaveReq{};
+        r.spp = 1;
+        r.sample_base = (int)padded;
+    }
+    WavePlan pl = make_plan((int)padded, max_level, fanout, n_on, n_pad);
+    if (!pl.no_sync) use_graph = false;
+    int st = host_buffers(ws);
+    if (st != RTG_OK) return st;
+    if (pl.no_sync && (st = reserve_all(ws, pl, A.cus > 0 ? A.cus : 256)) != RTG_OK) return st;
+    const size_t n_chunks = padded >> 6;
+    if (ws.grow(ws.b_reqs, reqs.size() * sizeof(WaveReq)) || ws.grow(ws.b_scenes, scenes.size() * sizeof(SceneView)) ||
+        ws.grow(ws.b_chunks, n_chunks * sizeof(int)) || ws.grow(ws.tables, table_floats * sizeof(float)))
+        return fail(RTG_E_NOMEM, "batch tables");
+    ws.tables_key = 0;  // the single-frame table cache no longer holds
+    // stage: ray-generation tables, request table, scene table
+    const size_t off_reqs = (table_floats * sizeof(float) + 255) & ~size_t(255);
+    const size_t off_scenes = (off_reqs + reqs.size() * sizeof(WaveReq) + 255) & ~size_t(255);
+    const size_t bytes = off_scenes + scenes.size() * sizeof(SceneView);
+    if ((st = staging(ws, bytes, stream)) != RTG_OK) return st;
+    RTG_HIP(hipStreamSynchronize(stream));  // the staging buffer may still feed the previous batch's copies
+    uint8_t *stg = static_cast<uint8_t *>(ws.staging);
+    float *tab = reinterpret_cast<float *>(stg);
+    size_t toff = 0;
+    for (int i = 0; i < n; ++i) {
+        const double dp = raygen_tables(items[i].scene->camera.fovy, items[i].W, items[i].H, tab + toff);
+        reqs[i].dpf = (float)dp;
+        reqs[i].ax = ws.tables.as<float>() + toff;
+        reqs[i].by = ws.tables.as<float>() + toff + items[i].W;
+        toff += (size_t)items[i].W + items[i].H;
+    }
+    std::memcpy(stg + off_reqs, reqs.data(), reqs.size() * sizeof(WaveReq));
+    for (size_t s = 0; s < scenes.size(); ++s)
+        std::memcpy(stg + off_scenes + s * sizeof(SceneView), &scenes[s]->view, sizeof(SceneView));
+    RTG_HIP(hipMemcpyAsync(ws.tables.p, stg, table_floats * sizeof(float), hipMemcpyHostToDevice, stream));
+    RTG_HIP(hipMemcpyAsync(ws.b_reqs.p, stg + off_reqs, reqs.size() * sizeof(WaveReq), hipMemcpyHostToDevice, stream));
+    RTG_HIP(hipMemcpyAsync(ws.b_scenes.p, stg + off_scenes, scenes.size() * sizeof(SceneView), hipMemcpyHostToDevice,
+                           stream));
+    WaveArgs a{};
+    a.reqs = ws.b_reqs.as<WaveReq>();
+    a.scenes = ws.b_scenes.as<SceneView>();
+    a.chunk_req = ws.b_chunks.as<int>();
+    a.n_samples = (int)padded;
+    a.counts = counts;
+    const uint64_t key = (uint64_t)padded ^ ((uint64_t)n_pad << 32) ^ ((uint64_t)max_level << 45) ^
+                         ((uint64_t)fanout << 50) ^ ((uint64_t)n_on << 53) ^ (ws.epoch * 0x9E3779B97F4A7C15ull) ^
+                         ((uint64_t)(uintptr_t)counts * 0xC2B2AE3D27D4EB4Full);
+    auto enqueue = [&]() -> int {
+        k_chunk_req<<<(int)((n_chunks + 255) / 256), 256, 0, stream>>>(a.reqs, n_pad, ws.b_chunks.as<int>(),
+                                                                      (int)n_chunks);
+        return enqueue_wave<true>(SceneView{}, a, pl, stream, ws, A, nullptr);
+    };
+    if (!use_graph) return enqueue();
+    if (ws.graph == nullptr || ws.graph_key != key) {
+        if (ws.graph) (void)hipGraphExecDestroy(ws.graph);
+        ws.graph = nullptr;
+        const bool kt = A.ktimes;
+        A.ktimes = false;  // no timing events inside a captured graph
+        RTG_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        st = enqueue();
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(stream, &g);
+        A.ktimes = kt;
+        if (st != RTG_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
+        if (e != hipSuccess) return fail(RTG_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        const hipError_t ei = hipGraphInstantiate(&ws.graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            ws.graph = nullptr;
+            return fail(RTG_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+        }
+        ws.graph_key = key;
+    } else if (graph_replayed) {
+        *graph_replayed = 1;
+    }
+    RTG_HIP(hipGraphLaunch(ws.graph, stream));
     return RTG_OK;
 }
 

@@ -232,4 +232,7 @@ int emul_render(const rtg_scene_desc *d, const rtg_params *p, int threads, const
 // High-water mark of the traversal stack over the last emul_render.
 int emul_max_stack(void) { return g_max_sp.load(); }
 
+// Level-0 length the batch engine pads `total` samples to (rt_prep.hpp).
+unsigned long long emul_batch_bucket(unsigned long long total) { return (unsigned long long)rtg::batch_bucket(total); }
+
 }  // extern "C"

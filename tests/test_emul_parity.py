@@ -89,3 +89,22 @@ def test_tile_compat_leaves_partial_tiles():
     o, e = _run(hs, p)
     assert (e["rgb"][128:] == 127).all() and (e["rgb"][:, 192:] == 127).all()
     assert (e["rgb"] == o["rgb"]).all()
+
+
+def test_batch_bucket_is_whole_chunks():
+    """Regression: the batch engine reads its level-0 queue in 64-sample chunks
+    through a chunk -> request table with one entry per chunk; a padded length
+    that is not a multiple of 64 (2,558,973 for a 2,325,760-sample batch) made
+    the last partial chunk read one entry past the table."""
+    import ctypes as C
+    from conftest import emul_lib
+    f = emul_lib().emul_batch_bucket
+    f.argtypes, f.restype = [C.c_ulonglong], C.c_ulonglong
+    rng = np.random.default_rng(0)
+    totals = [0, 1, 64, 4095, 4096, 4097, 46656, 2325760, 2558973, 10 ** 8] + list(rng.integers(1, 2 ** 30, 2000))
+    for t in totals:
+        p = f(int(t))
+        assert p % 4096 == 0 and p >= t, (t, p)
+        assert p <= max(4096, t + t // 8 + 4096), (t, p)
+    # batches of similar size share a bucket (and so a captured graph)
+    assert f(2325760) == f(2325760 + 64)
