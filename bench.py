@@ -62,6 +62,89 @@ def pmc_traffic(engine: str, kernel: str):
         return None
 
 
+def batch_scenes(rt):
+    """The 8 resident scenes of the batch and serving workloads (triangle soups
+    and height fields of 1k..1M triangles, 1-8 lights, depth 2-6)."""
+    hs = [rt.HostScene.workload(3, index=i) for i in range(8)]
+    return hs, [rt.DeviceScene(h) for h in hs]
+
+
+def bench_batch(args) -> dict:
+    """Workload 3: 256 requests (64x64 .. 1024x768) over the 8 resident scenes,
+    rendered as one batch (one wavefront, replayed from a captured hipGraph),
+    against the plain renderer (one frame per request)."""
+    import raytrace_amd as rt
+    rt.init(0)
+    hs, ds = batch_scenes(rt)
+    reqs = rt.gen_requests(256, 0.0, seed=3)
+    items = [(ds[s], rt.params(w, h, max_level=hs[s].info.max_level)) for s, w, h, _ in reqs]
+    for _ in range(max(2, args.warmup)):
+        _, st = rt.render_batch(items, ids=False)
+    t0 = time.perf_counter()
+    replays = 0
+    for _ in range(args.steps):
+        _, st = rt.render_batch(items, ids=False)
+        replays += st.graph_replays
+    batch_s = (time.perf_counter() - t0) / args.steps
+    rays = st.rays_total
+    for d, p in items[:8]:
+        d.render(p, ids=False, tprim=False)  # warm the single-frame path
+    t0 = time.perf_counter()
+    prays = 0
+    for d, p in items:
+        prays += d.render(p, ids=False, tprim=False)["stats"].rays_total
+    plain_s = time.perf_counter() - t0
+    assert prays == rays
+    return {
+        "metric": "Mrays/s (primary+secondary), 256 batched requests", "value": round(rays / batch_s / 1e6, 2),
+        "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(batch_s * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "256 requests 64x64..1024x768 over 8 resident scenes (1k..1M triangles, 1-8 lights, "
+                               "depth 2-6), RGB to host", "rays_per_batch": rays,
+                   "pixels_per_batch": sum(p.width * p.height for _, p in items),
+                   "scene_triangles": [int(h.info.n_tris) for h in hs], "graph_replayed_steps": replays},
+        "plain": {"ms_per_batch": round(plain_s * 1e3, 3), "mrays_per_s": round(rays / plain_s / 1e6, 2),
+                  "what": "the same 256 requests, one rtg_render each"},
+        "speedup_vs_plain": round(plain_s / batch_s, 2),
+    }
+
+
+def bench_serve(args) -> dict:
+    """Workload 4: Poisson arrivals at each rate for `--duration` seconds over the
+    8 resident scenes; the continuous-batching loop against the plain renderer
+    (one frame per request, first come first served) on the same arrivals."""
+    import raytrace_amd as rt
+    rt.init(0)
+    hs, ds = batch_scenes(rt)
+    rates = [float(r) for r in args.rates.split(",")]
+    rows = []
+    for rate in rates:
+        n = max(8, int(rate * args.duration))
+        reqs = rt.gen_requests(n, rate, seed=4)
+        items = [(ds[s], rt.params(w, h, max_level=hs[s].info.max_level), t) for s, w, h, t in reqs]
+        row = {"rate": rate, "requests": n}
+        for name, mode in (("batched", rt._capi.RTG_SERVE_BATCHED), ("plain", rt._capi.RTG_SERVE_PLAIN)):
+            _, _, st = rt.serve(items, mode=mode, slots=args.slots)
+            row[name] = {"p50_ms": round(st.p50_ms, 3), "p99_ms": round(st.p99_ms, 3),
+                         "req_per_s": round(st.req_per_s, 1), "mrays_per_s": round(st.mrays_per_s, 1),
+                         "batches": st.batches, "max_in_flight": st.max_in_flight}
+            sys.stderr.write(f"rate {rate:g}/s {name}: {row[name]}\n")
+            sys.stderr.flush()
+        rows.append(row)
+    top = rows[-1]
+    return {
+        "metric": "serve-loop p50/p99 frame latency and throughput", "value": top["batched"]["req_per_s"],
+        "unit": "requests/s at the highest offered rate", "n_gpus": 1, "steps": len(rates), "warmup": 0,
+        "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"Poisson arrivals at {args.rates} req/s, {args.duration:g} s each, requests 64x64.."
+                               "1024x768 over 8 resident scenes, RGB to host", "slots": args.slots},
+        "rates": rows,
+        "throughput_vs_plain_at_top_rate": round(top["batched"]["req_per_s"] / max(1e-9, top["plain"]["req_per_s"]), 2),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +153,11 @@ def main() -> int:
     ap.add_argument("--engine", default="auto", choices=["mega", "wave", "auto"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-tiles", type=int, default=32, help="1/N of the 64x64 tiles for the CPU baseline sample")
+    ap.add_argument("--workload", type=int, default=2, choices=[2, 3, 4],
+                    help="2: the 1080p frame (default line); 3: 256 batched requests; 4: serving sweep")
+    ap.add_argument("--rates", default="50,100,200,500,1000,2000", help="workload 4: offered rates (requests/s)")
+    ap.add_argument("--duration", type=float, default=2.0, help="workload 4: seconds of arrivals per rate")
+    ap.add_argument("--slots", type=int, default=2, help="workload 4: concurrent batches")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (reference for FETCH_SIZE / WRITE_SIZE scaling)")
     args = ap.parse_args()
@@ -86,6 +174,12 @@ def main() -> int:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(frames and requests shard across GPUs without exchange; see the design notes)", file=sys.stderr)
         return 2
+    if args.workload == 3:
+        print(json.dumps(bench_batch(args)))
+        return 0
+    if args.workload == 4:
+        print(json.dumps(bench_serve(args)))
+        return 0
 
     import numpy as np
     import torch

@@ -282,6 +282,51 @@ typedef struct rtg_request {
  * counterpart in the reference, which renders one frame per start(). */
 int rtg_render_batch(const rtg_request *reqs, int n, rtg_stats *stats);
 
+/* ---- serving loop ----------------------------------------------------- */
+
+/* One request of a serving run: a frame of a resident scene, arriving at
+ * `arrival_s` seconds after the run starts; RGB lands in `rgb` (host,
+ * W*H*3).  No counterpart in the reference, which renders one frame per
+ * start() call and has no request loop. */
+typedef struct rtg_serve_request {
+    rtg_scene_t scene;
+    rtg_params params;
+    uint8_t *rgb;
+    double arrival_s;
+    double done_s;   /* out: seconds after the start when the RGB was in host memory */
+    int32_t status;  /* out: RTG_OK or an error code */
+    int32_t batch;   /* out: which batch rendered it */
+} rtg_serve_request;
+
+#define RTG_SERVE_BATCHED 0 /* continuous batching over concurrent slots */
+#define RTG_SERVE_PLAIN 1   /* one render per request, first come first served */
+
+typedef struct rtg_serve_opts {
+    int32_t mode;        /* RTG_SERVE_* */
+    int32_t slots;       /* batches in flight at once, each on its own stream (0: 2) */
+    int32_t max_batch;   /* requests per batch (0: 256, at most 4096) */
+    int32_t pad;
+    uint64_t hbm_budget; /* queue bytes one batch may reserve (0: 4 GiB); admission stops there */
+} rtg_serve_opts;
+
+typedef struct rtg_serve_stats {
+    double wall_s;                       /* first arrival to last completion */
+    double p50_ms, p99_ms, mean_ms, max_ms; /* latency: arrival -> RGB in host memory */
+    double mrays_per_s, req_per_s;
+    uint64_t rays_total;                 /* reference-semantic rays of all requests */
+    int32_t batches, max_in_flight;      /* batches launched; most requests on the GPU at once */
+} rtg_serve_stats;
+
+/* Serve n requests (open loop: each is admitted once its arrival time has
+ * passed).  RTG_SERVE_BATCHED: whenever a slot is free, every request that
+ * has arrived and is waiting is admitted into one batch, up to max_batch and
+ * the HBM budget, and rendered as one wavefront (rtg_render_batch's engine)
+ * while the other slots' batches run; requests that arrive meanwhile wait
+ * for the next free slot.  RTG_SERVE_PLAIN: the same arrivals rendered one
+ * request at a time, each as its own frame.  Each request's RGB is identical
+ * to rtg_render of it alone. */
+int rtg_serve(rtg_serve_request *reqs, int n, const rtg_serve_opts *opts, rtg_serve_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

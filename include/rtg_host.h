@@ -42,6 +42,19 @@ typedef struct rtg_gen_info {
  * 200k-triangle icosphere + 64 spheres; 3: one of the 8 batch scenes; 5: the
  * 4K terrain stress scene) through the Scene API. */
 int rtg_host_scene_config(int config, const rtg_gen_opts *opts, rtg_host_scene_t *out, rtg_gen_info *info);
+/* One request of the batch and serving workloads (3 and 4). */
+typedef struct rtg_gen_request {
+    int32_t scene;          /* which of the 8 resident scenes (rtg_gen_opts.index) */
+    int32_t width, height;  /* 64x64 .. 1024x768, multiples of 64 */
+    int32_t pad;
+    double arrival_s;       /* workload 4: Poisson arrival time; workload 3: 0 */
+} rtg_gen_request;
+
+/* Request stream of workloads 3 and 4 (seeded splitmix64, u = (x >> 40) 2^-24):
+ * request k draws scene floor(8u), width 64 round(16^u), height 64 round(12^u);
+ * with rate > 0 arrivals are Poisson at `rate` requests per second. */
+int rtg_host_gen_requests(uint64_t seed, int n, double rate, rtg_gen_request *out);
+
 /* Scene::Scene (Scene.cpp:6-67): default camera, EnvLight and material library. */
 int rtg_host_scene_new(rtg_host_scene_t *out);
 int rtg_host_scene_destroy(rtg_host_scene_t scene);

@@ -213,3 +213,32 @@ def render_batch(items, ids: bool = True, eager: bool = False):
     st = rtg_stats()
     check(lib.rtg_render_batch(reqs, n, C.byref(st)), "rtg_render_batch")
     return outs, st
+
+
+def gen_requests(n: int, rate: float = 0.0, seed: int = 3):
+    """Request stream of workloads 3/4: [(scene index, width, height, arrival_s)]."""
+    lib = _capi.load()
+    buf = (_capi.rtg_gen_request * max(1, n))()
+    check(lib.rtg_host_gen_requests(seed, n, rate, buf), "rtg_host_gen_requests")
+    return [(buf[i].scene, buf[i].width, buf[i].height, buf[i].arrival_s) for i in range(n)]
+
+
+def serve(items, mode: int = _capi.RTG_SERVE_BATCHED, slots: int = 0, max_batch: int = 0, hbm_budget: int = 0):
+    """rtg_serve over [(DeviceScene, rtg_params, arrival_s), ...].
+    Returns ([rgb per request], [(done_s, batch) per request], rtg_serve_stats)."""
+    lib = _capi.load()
+    n = len(items)
+    reqs = (_capi.rtg_serve_request * max(1, n))()
+    outs = []
+    for k, (ds, p, t) in enumerate(items):
+        rgb = np.empty((p.height, p.width, 3), np.uint8)
+        reqs[k].scene = ds.h
+        reqs[k].params = p
+        reqs[k].rgb = rgb.ctypes.data_as(C.POINTER(C.c_uint8))
+        reqs[k].arrival_s = t
+        outs.append(rgb)
+    o = _capi.rtg_serve_opts(mode=mode, slots=slots, max_batch=max_batch, hbm_budget=hbm_budget)
+    st = _capi.rtg_serve_stats()
+    check(lib.rtg_serve(reqs, n, C.byref(o), C.byref(st)), "rtg_serve")
+    return outs, [(reqs[k].done_s, reqs[k].batch) for k in range(n)], st
+

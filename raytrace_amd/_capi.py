@@ -28,6 +28,8 @@ RTG_ENGINE_WAVEFRONT = 2
 RTG_FLAG_FP64_RETEST = 0x1
 RTG_FLAG_KERNEL_TIMES = 0x2
 RTG_FLAG_EAGER = 0x4
+RTG_SERVE_BATCHED = 0
+RTG_SERVE_PLAIN = 1
 
 
 class rtg_vec4(C.Structure):
@@ -105,6 +107,27 @@ class rtg_params(C.Structure):
     ]
 
 
+class rtg_serve_request(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("params", rtg_params), ("rgb", C.POINTER(C.c_uint8)),
+                ("arrival_s", C.c_double), ("done_s", C.c_double), ("status", C.c_int32), ("batch", C.c_int32)]
+
+
+class rtg_serve_opts(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("slots", C.c_int32), ("max_batch", C.c_int32), ("pad", C.c_int32),
+                ("hbm_budget", C.c_uint64)]
+
+
+class rtg_serve_stats(C.Structure):
+    _fields_ = [("wall_s", C.c_double), ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double),
+                ("max_ms", C.c_double), ("mrays_per_s", C.c_double), ("req_per_s", C.c_double),
+                ("rays_total", C.c_uint64), ("batches", C.c_int32), ("max_in_flight", C.c_int32)]
+
+
+class rtg_gen_request(C.Structure):
+    _fields_ = [("scene", C.c_int32), ("width", C.c_int32), ("height", C.c_int32), ("pad", C.c_int32),
+                ("arrival_s", C.c_double)]
+
+
 class rtg_kernel_time(C.Structure):
     _fields_ = [("name", C.c_char * 24), ("launches", C.c_uint32), ("pad", C.c_uint32), ("items", C.c_uint64),
                 ("ms", C.c_double)]
@@ -165,9 +188,11 @@ SIGNATURES = {
     "rtg_job_free": (_I, [_P]),
     "rtg_render_batch": (_I, [C.POINTER(rtg_request), _I, C.POINTER(rtg_stats)]),
     "rtg_kernel_times": (_I, [C.POINTER(rtg_kernel_time), _I, C.POINTER(_I)]),
+    "rtg_serve": (_I, [C.POINTER(rtg_serve_request), _I, C.POINTER(rtg_serve_opts), C.POINTER(rtg_serve_stats)]),
     # rtg_host.h
     "rtg_host_scene_config": (_I, [_I, C.POINTER(rtg_gen_opts), C.POINTER(_P), C.POINTER(rtg_gen_info)]),
     "rtg_host_scene_new": (_I, [C.POINTER(_P)]),
+    "rtg_host_gen_requests": (_I, [C.c_uint64, _I, C.c_double, C.POINTER(rtg_gen_request)]),
     "rtg_host_scene_destroy": (_I, [_P]),
     "rtg_host_scene_desc": (_P, [_P]),
     "rtg_host_add_sphere": (_I, [_P, _F]),

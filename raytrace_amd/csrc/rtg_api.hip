@@ -13,12 +13,6 @@ thread_local std::string g_err = "";
 std::mutex g_init_lock;
 std::unique_ptr<Arena> g_arena;
 
-__global__ void k_fill_u8(uint8_t *p, size_t n, uint8_t v) {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
-}
-__global__ void k_zero_u64(unsigned long long *p, int n) {
-    if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
-}
 
 uint64_t hash_bytes(uint64_t h, const void *p, size_t n) {
     const uint8_t *c = static_cast<const uint8_t *>(p);
@@ -64,6 +58,15 @@ int validate_desc(const rtg_scene_desc *d) {
     return RTG_OK;
 }
 
+
+}  // namespace
+
+void set_error(const std::string &msg) { g_err = msg; }
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
 int resolve_size(const SceneRes &S, const rtg_params *p, int &W, int &H) {
     if (p == nullptr) return fail(RTG_E_INVALID, "params is NULL");
     W = p->width > 0 ? p->width : S.camera.width;
@@ -74,12 +77,11 @@ int resolve_size(const SceneRes &S, const rtg_params *p, int &W, int &H) {
     return RTG_OK;
 }
 
-}  // namespace
-
-void set_error(const std::string &msg) { g_err = msg; }
-int fail(int code, const std::string &msg) {
-    g_err = msg;
-    return code;
+__global__ void k_fill_u8(uint8_t *p, size_t n, uint8_t v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_zero_u64(unsigned long long *p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
 }
 
 int DevBuf::alloc(size_t n) {
@@ -191,7 +193,7 @@ static int render_device(SceneRes &S, const rtg_params &p, int W, int H, const R
     int engine = p.engine == RTG_ENGINE_AUTO ? RTG_ENGINE_WAVEFRONT : p.engine;
     int st = A.ws.order_before(stream);
     if (st != RTG_OK) return st;
-    st = engine == RTG_ENGINE_WAVEFRONT ? launch_wavefront(S, p, W, H, o, stream, A)
+    st = engine == RTG_ENGINE_WAVEFRONT ? launch_wavefront(S, p, W, H, o, stream, A.ws, A, A.cancel_host)
                                         : launch_megakernel(S, p, W, H, o, stream, A);
     const int st2 = A.ws.mark_done(stream);
     return st != RTG_OK ? st : st2;
@@ -200,10 +202,6 @@ static int render_device(SceneRes &S, const rtg_params &p, int W, int H, const R
 }  // namespace rtg
 
 using namespace rtg;
-
-struct rtg_scene_s {
-    SceneRes res;
-};
 
 struct rtg_job_s {
     std::thread th;
@@ -249,6 +247,7 @@ int rtg_shutdown(void) {
     {
         std::lock_guard<std::mutex> l(A.lock);
         (void)hipStreamSynchronize(A.stream);
+        A.serve_slots.clear();
         A.ws.release();
         A.counters.release();
         A.out_rgb.release();
@@ -532,7 +531,7 @@ int rtg_render_batch(const rtg_request *reqs, int n, rtg_stats *stats) {
     RTG_HIP(hipEventRecord(A.ev0, s));
     int replayed = 0;
     int st = A.ws.order_before(s);
-    if (st == RTG_OK) st = launch_batch(items.data(), n, counts, s, A.ws, A, !eager, &replayed);
+    if (st == RTG_OK) st = launch_batch(items.data(), n, counts, s, A.ws, A, eager ? GRAPH_NEVER : GRAPH_FIRST, &replayed);
     if (st == RTG_OK) st = A.ws.mark_done(s);
     RTG_HIP(hipEventRecord(A.ev1, s));
     unsigned long long cnt[4] = {0, 0, 0, 0};

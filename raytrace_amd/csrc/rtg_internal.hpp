@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <memory>
 #include <condition_variable>
 #include <mutex>
 #include <string>
@@ -66,6 +67,8 @@ struct Workspace {
     uint64_t epoch = 0;  // bumped when a buffer moves (captured graphs hold addresses)
     hipGraphExec_t graph = nullptr;
     uint64_t graph_key = 0;
+    uint64_t seen_keys[8] = {};  // batch shapes launched recently (GRAPH_SECOND)
+    int seen_next = 0;
     // Recorded after the last render enqueued with this workspace: a render
     // on another stream waits for it (device-side) before reusing the buffers.
     hipEvent_t done = nullptr;
@@ -77,8 +80,27 @@ struct Workspace {
     ~Workspace() { release(); }
 };
 
+// One concurrent batch of the serving loop: its stream, workspace, device
+// outputs and pinned host staging, and the requests it carries.
+struct ServeSlot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    Workspace ws;
+    DevBuf out, counts;
+    uint8_t *pinned = nullptr;
+    size_t pinned_bytes = 0;
+    unsigned long long *counts_host = nullptr;  // pinned, 4
+    std::vector<int> reqs;
+    std::vector<size_t> offs;
+    bool busy = false;
+    ServeSlot() = default;
+    ServeSlot(const ServeSlot &) = delete;
+    ServeSlot &operator=(const ServeSlot &) = delete;
+    ~ServeSlot();
+};
+
 // Per-device render state: default stream and workspace, outputs, cancel
-// flag, kernel timing.
+// flag, kernel timing, serving slots.
 struct Arena {
     static constexpr int kLevels = Workspace::kLevels;
     int device = -1;
@@ -87,6 +109,7 @@ struct Arena {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBuf counters, out_rgb, out_ids, out_t;
     Workspace ws;
+    std::vector<std::unique_ptr<ServeSlot>> serve_slots;
     int *cancel_host = nullptr;  // pinned, mapped
     int *cancel_dev = nullptr;
     // kernel timing (RTG_FLAG_KERNEL_TIMES)
@@ -123,6 +146,9 @@ Arena *arena();
 int ensure(DevBuf &b, size_t bytes);
 
 int scene_build(SceneRes &S, const rtg_scene_desc &d, hipStream_t stream);
+// Frame size of a render (params or the scene camera's) and parameter checks.
+int resolve_size(const SceneRes &S, const rtg_params *p, int &W, int &H);
+
 
 struct RenderOut {
     uint8_t *rgb;   // device
@@ -134,9 +160,10 @@ struct RenderOut {
 // Launch one frame of the megakernel on `stream` (asynchronous).
 int launch_megakernel(const SceneRes &S, const rtg_params &p, int W, int H, const RenderOut &o, hipStream_t stream,
                       Arena &A);
-// Launch one frame of the wavefront engine on `stream` (asynchronous).
+// Launch one frame of the wavefront engine on `stream` (asynchronous) with
+// the buffers of `ws`; `cancel_host` (optional) is polled between levels.
 int launch_wavefront(const SceneRes &S, const rtg_params &p, int W, int H, const RenderOut &o, hipStream_t stream,
-                     Arena &A);
+                     Workspace &ws, Arena &A, const int *cancel_host);
 
 // One request of a batch (rtg_render_batch): a frame of `scene` into device
 // outputs.
@@ -150,10 +177,23 @@ struct BatchItem {
 };
 // Enqueue every request of a batch on `stream` as one wavefront: all their
 // rays share one queue per level; no host round trip when the queue bounds fit
-// the workspace budget.  With `use_graph`, the launch sequence is captured
-// into a hipGraph and replayed for later batches of the same shape.
+// the workspace budget.  The launch sequence is captured into a hipGraph and
+// replayed for later batches of the same padded shape: GRAPH_NEVER,
+// GRAPH_FIRST (capture at the first launch of a shape) or GRAPH_SECOND
+// (capture when a shape comes back; one-off shapes run eagerly).
 // `counts` (device, 4 x u64) accumulates the ray counts of the whole batch.
+enum GraphPolicy { GRAPH_NEVER = 0, GRAPH_FIRST = 1, GRAPH_SECOND = 2 };
 int launch_batch(const BatchItem *items, int n, unsigned long long *counts, hipStream_t stream, Workspace &ws,
-                 Arena &A, bool use_graph, int *graph_replayed);
+                 Arena &A, int graph_policy, int *graph_replayed);
+// Device bytes the queues of a batch reserve (its no-sync bound), for
+// admission by memory budget; 0 when the bound exceeds the no-sync budget.
+size_t batch_queue_bytes(size_t samples, int max_level, int fanout, int n_on);
+// Level-0 queue length of a single frame.
+inline size_t frame_samples(int W, int H, int spp) { return (size_t)((W + 7) / 8) * ((H + 7) / 8) * 64 * (spp > 0 ? spp : 1); }
 
 }  // namespace rtg
+
+// Opaque scene handle of the C ABI.
+struct rtg_scene_s {
+    rtg::SceneRes res;
+};

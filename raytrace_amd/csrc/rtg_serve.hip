@@ -1,0 +1,266 @@
+// rtg_serve.hip — the request loop: open-loop arrivals, admission by HBM
+// budget, continuous batching over concurrent slots (rtg_serve).
+//
+// Batched mode.  A slot is a stream with its own workspace (queues, spill,
+// tables, captured graphs), device output buffer and pinned host staging.
+// The loop admits every request whose arrival time has passed; whenever a
+// slot is free it forms one batch of waiting requests (first come first
+// served, up to max_batch and the HBM budget of the batch's queues) and
+// enqueues it as one wavefront on that slot, followed by the copy of the
+// batch's RGB to pinned memory and an event.  Several slots run at once, so
+// a batch admitted while others are on the GPU overlaps them on the device.
+// A completed slot's RGB is copied into each request's buffer and the
+// request is done.  Shapes that come back are replayed from captured graphs.
+//
+// Plain mode.  One slot, one request at a time, each rendered as its own
+// frame (the single-frame wavefront), in arrival order: the plain renderer
+// the loop is compared with.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <thread>
+
+#include "rt_prep.hpp"
+#include "rtg_internal.hpp"
+
+namespace rtg {
+namespace {
+
+__global__ void k_fill_bytes(uint8_t *p, size_t n, uint8_t v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_zero_counts(unsigned long long *p) {
+    if (threadIdx.x < 4) p[threadIdx.x] = 0;
+}
+
+using Clock = std::chrono::steady_clock;
+
+double since(Clock::time_point t0) { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+
+int slot_init(ServeSlot &s) {
+    if (s.stream == nullptr) RTG_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+    if (s.done == nullptr) RTG_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+    if (s.counts_host == nullptr)
+        RTG_HIP(hipHostMalloc(reinterpret_cast<void **>(&s.counts_host), 4 * sizeof(unsigned long long),
+                              hipHostMallocDefault));
+    if (ensure(s.counts, 4 * sizeof(unsigned long long))) return fail(RTG_E_NOMEM, "serve counters");
+    return RTG_OK;
+}
+
+int slot_reserve(ServeSlot &s, size_t bytes) {
+    if (ensure(s.out, bytes)) return fail(RTG_E_NOMEM, "serve outputs");
+    if (s.pinned_bytes < bytes) {
+        if (s.pinned) (void)hipHostFree(s.pinned);
+        s.pinned = nullptr;
+        s.pinned_bytes = 0;
+        const size_t n = bytes + bytes / 4;
+        RTG_HIP(hipHostMalloc(reinterpret_cast<void **>(&s.pinned), n, hipHostMallocDefault));
+        s.pinned_bytes = n;
+    }
+    return RTG_OK;
+}
+
+struct ReqInfo {
+    int W, H;
+    size_t samples;
+    int fanout, n_on;
+};
+
+}  // namespace
+
+ServeSlot::~ServeSlot() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    ws.release();
+    if (pinned) (void)hipHostFree(pinned);
+    if (counts_host) (void)hipHostFree(counts_host);
+    if (done) (void)hipEventDestroy(done);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+// Enqueue `ids` as one batch on slot `s` (asynchronous; `s.done` marks the
+// end of the RGB copy to pinned memory).
+static int slot_launch(ServeSlot &s, rtg_serve_request *reqs, const std::vector<ReqInfo> &info,
+                       const std::vector<int> &ids, bool plain, Arena &A) {
+    size_t bytes = 0;
+    s.offs.resize(ids.size());
+    for (size_t k = 0; k < ids.size(); ++k) {
+        s.offs[k] = bytes;
+        bytes += ((size_t)info[ids[k]].W * info[ids[k]].H * 3 + 255) & ~size_t(255);
+    }
+    int st = slot_reserve(s, bytes);
+    if (st != RTG_OK) return st;
+    k_fill_bytes<<<1024, 256, 0, s.stream>>>(s.out.as<uint8_t>(), bytes, 127);
+    k_zero_counts<<<1, 64, 0, s.stream>>>(s.counts.as<unsigned long long>());
+    if (plain) {
+        const rtg_serve_request &q = reqs[ids[0]];
+        const RenderOut o{s.out.as<uint8_t>(), nullptr, nullptr, s.counts.as<unsigned long long>()};
+        st = launch_wavefront(q.scene->res, q.params, info[ids[0]].W, info[ids[0]].H, o, s.stream, s.ws, A, nullptr);
+    } else {
+        std::vector<BatchItem> items(ids.size());
+        for (size_t k = 0; k < ids.size(); ++k) {
+            const rtg_serve_request &q = reqs[ids[k]];
+            items[k] = BatchItem{&q.scene->res, q.params, info[ids[k]].W, info[ids[k]].H,
+                                 s.out.as<uint8_t>() + s.offs[k], nullptr, nullptr};
+        }
+        int replayed = 0;
+        st = launch_batch(items.data(), (int)items.size(), s.counts.as<unsigned long long>(), s.stream, s.ws, A,
+                          GRAPH_SECOND, &replayed);
+    }
+    if (st != RTG_OK) return st;
+    RTG_HIP(hipMemcpyAsync(s.pinned, s.out.p, bytes, hipMemcpyDeviceToHost, s.stream));
+    RTG_HIP(hipMemcpyAsync(s.counts_host, s.counts.p, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           s.stream));
+    RTG_HIP(hipEventRecord(s.done, s.stream));
+    s.reqs = ids;
+    s.busy = true;
+    return RTG_OK;
+}
+
+int serve_run(rtg_serve_request *reqs, int n, const rtg_serve_opts &o, rtg_serve_stats &out, Arena &A) {
+    const bool plain = o.mode == RTG_SERVE_PLAIN;
+    const int nslots = plain ? 1 : (o.slots > 0 ? std::min(o.slots, 8) : 2);
+    const int max_batch = plain ? 1 : (o.max_batch > 0 ? std::min(o.max_batch, 4096) : 256);
+    const size_t budget = o.hbm_budget > 0 ? (size_t)o.hbm_budget : (size_t(4) << 30);
+    std::vector<ReqInfo> info(n);
+    for (int i = 0; i < n; ++i) {
+        rtg_serve_request &q = reqs[i];
+        q.status = RTG_E_PENDING;
+        q.done_s = 0.0;
+        q.batch = -1;
+        if (q.scene == nullptr || q.rgb == nullptr) return fail(RTG_E_INVALID, "request without scene or rgb");
+        int st = resolve_size(q.scene->res, &q.params, info[i].W, info[i].H);
+        if (st != RTG_OK) return st;
+        if (q.params.mode != RTG_MODE_RAYTRACE && q.params.mode != RTG_MODE_REFRACTTEST && q.params.mode != 0)
+            return fail(RTG_E_INVALID, "render mode not supported by this build");
+        info[i].samples = frame_samples(info[i].W, info[i].H, q.params.spp);
+        info[i].fanout = q.scene->res.max_fanout;
+        info[i].n_on = __builtin_popcount(q.scene->res.lights_on);
+    }
+    while ((int)A.serve_slots.size() < nslots) A.serve_slots.emplace_back(new ServeSlot());
+    for (int k = 0; k < nslots; ++k) {
+        const int st = slot_init(*A.serve_slots[k]);
+        if (st != RTG_OK) return st;
+    }
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return reqs[a].arrival_s < reqs[b].arrival_s; });
+
+    std::vector<int> waiting;  // arrived, not yet admitted (arrival order)
+    size_t head = 0;           // first waiting entry not yet admitted
+    int next = 0, completed = 0, batches = 0, in_flight = 0, max_in_flight = 0, st = RTG_OK;
+    uint64_t rays = 0;
+    const Clock::time_point t0 = Clock::now();
+    while (completed < n) {
+        bool progress = false;
+        double now = since(t0);
+        while (next < n && reqs[order[next]].arrival_s <= now) waiting.push_back(order[next++]);
+        // completions
+        for (int k = 0; k < nslots; ++k) {
+            ServeSlot &s = *A.serve_slots[k];
+            if (!s.busy) continue;
+            const hipError_t e = hipEventQuery(s.done);
+            if (e == hipErrorNotReady) continue;
+            if (e != hipSuccess) {
+                st = fail(RTG_E_DEVICE, std::string("serve batch: ") + hipGetErrorString(e));
+                break;
+            }
+            for (size_t j = 0; j < s.reqs.size(); ++j) {
+                rtg_serve_request &q = reqs[s.reqs[j]];
+                std::memcpy(q.rgb, s.pinned + s.offs[j], (size_t)info[s.reqs[j]].W * info[s.reqs[j]].H * 3);
+            }
+            const double t = since(t0);
+            for (int id : s.reqs) {
+                reqs[id].done_s = t;
+                reqs[id].status = RTG_OK;
+            }
+            rays += s.counts_host[0] + s.counts_host[1] + s.counts_host[2] + s.counts_host[3];
+            completed += (int)s.reqs.size();
+            in_flight -= (int)s.reqs.size();
+            s.busy = false;
+            progress = true;
+        }
+        if (st != RTG_OK) break;
+        // admissions: fill free slots with waiting requests
+        for (int k = 0; k < nslots && head < waiting.size(); ++k) {
+            ServeSlot &s = *A.serve_slots[k];
+            if (s.busy) continue;
+            std::vector<int> ids;
+            size_t samples = 0;
+            int lvl = 0, fan = 0, on = 0;
+            while (head < waiting.size() && (int)ids.size() < max_batch) {
+                const int id = waiting[head];
+                const ReqInfo &r = info[id];
+                const int l2 = std::max(lvl, (int)reqs[id].params.max_level), f2 = std::max(fan, r.fanout),
+                          o2 = std::max(on, r.n_on);
+                const size_t need = batch_queue_bytes(batch_bucket(samples + r.samples), l2, f2, o2);
+                if (!ids.empty() && (need == 0 || need > budget)) break;  // the next batch takes it
+                ids.push_back(id);
+                samples += r.samples, lvl = l2, fan = f2, on = o2;
+                ++head;
+            }
+            st = slot_launch(s, reqs, info, ids, plain, A);
+            if (st != RTG_OK) break;
+            for (int id : ids) reqs[id].batch = batches;
+            ++batches;
+            in_flight += (int)ids.size();
+            max_in_flight = std::max(max_in_flight, in_flight);
+            progress = true;
+        }
+        if (st != RTG_OK) break;
+        if (!progress) {
+            // nothing to do until a slot completes or the next request arrives
+            double wait = 50e-6;
+            if (head >= waiting.size() && next < n && in_flight == 0)
+                wait = std::max(0.0, std::min(reqs[order[next]].arrival_s - since(t0), 2e-3));
+            if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
+        }
+    }
+    if (st != RTG_OK) {
+        for (int k = 0; k < nslots; ++k) (void)hipStreamSynchronize(A.serve_slots[k]->stream);
+        for (auto &s : A.serve_slots) s->busy = false;
+        return st;
+    }
+    std::vector<double> lat(n);
+    double first = 1e300, last = 0.0, sum = 0.0, mx = 0.0;
+    for (int i = 0; i < n; ++i) {
+        lat[i] = (reqs[i].done_s - reqs[i].arrival_s) * 1e3;
+        first = std::min(first, reqs[i].arrival_s);
+        last = std::max(last, reqs[i].done_s);
+        sum += lat[i];
+        mx = std::max(mx, lat[i]);
+    }
+    std::sort(lat.begin(), lat.end());
+    auto pct = [&](double p) { return n ? lat[std::min(n - 1, (int)std::ceil(p * n) - 1)] : 0.0; };
+    out = rtg_serve_stats{};
+    out.wall_s = n ? last - first : 0.0;
+    out.p50_ms = pct(0.50);
+    out.p99_ms = pct(0.99);
+    out.mean_ms = n ? sum / n : 0.0;
+    out.max_ms = mx;
+    out.rays_total = rays;
+    out.mrays_per_s = out.wall_s > 0 ? rays / out.wall_s / 1e6 : 0.0;
+    out.req_per_s = out.wall_s > 0 ? n / out.wall_s : 0.0;
+    out.batches = batches;
+    out.max_in_flight = max_in_flight;
+    return RTG_OK;
+}
+
+}  // namespace rtg
+
+using namespace rtg;
+
+extern "C" int rtg_serve(rtg_serve_request *reqs, int n, const rtg_serve_opts *opts, rtg_serve_stats *stats) {
+    if (n < 0 || (n > 0 && reqs == nullptr)) return fail(RTG_E_INVALID, "bad request list");
+    if (!arena()) return fail(RTG_E_STATE, "call rtg_init first");
+    rtg_serve_opts o{};
+    if (opts) o = *opts;
+    if (o.mode != RTG_SERVE_BATCHED && o.mode != RTG_SERVE_PLAIN) return fail(RTG_E_INVALID, "unknown serve mode");
+    Arena &A = *arena();
+    std::lock_guard<std::mutex> l(A.lock);
+    A.ktimes = false;
+    rtg_serve_stats s{};
+    const int st = serve_run(reqs, n, o, s, A);
+    if (stats) *stats = s;
+    return st;
+}

@@ -470,6 +470,8 @@ __global__ void k_chunk_req(const WaveReq *reqs, int n_reqs, int *chunk_req, int
 // Largest queue memory (bytes) a frame may reserve to run without host round
 // trips between levels.
 constexpr size_t kNoSyncBudget = size_t(8) << 30;
+// Single frames with more samples run level by level (see launch_wavefront).
+constexpr int kNoSyncMaxSamples = 1 << 20;
 
 // Resident blocks per CU of a kernel at kBlockThreads threads (registers and
 // LDS permitting).
@@ -501,11 +503,25 @@ struct WavePlan {
     bool no_sync;
 };
 
+static size_t per_ray_bytes(int n_on) {
+    return sizeof(WRay) + sizeof(WHit) + sizeof(WNode) + sizeof(unsigned) + 2 * sizeof(float4) +
+           (size_t)n_on * sizeof(unsigned);
+}
+
+size_t batch_queue_bytes(size_t samples, int max_level, int fanout, int n_on) {
+    size_t need = 0, b = samples;
+    for (int k = 0; k <= max_level && b > 0; ++k) {
+        need += b * per_ray_bytes(n_on);
+        if (need > kNoSyncBudget) return 0;
+        b *= (size_t)fanout;
+    }
+    return need;
+}
+
 static WavePlan make_plan(int n_samples, int max_level, int fanout, int n_on, int n_reqs) {
     WavePlan pl{};
     pl.n_samples = n_samples, pl.max_level = max_level, pl.fanout = fanout, pl.n_on = n_on, pl.n_reqs = n_reqs;
-    const size_t per_ray = sizeof(WRay) + sizeof(WHit) + sizeof(WNode) + sizeof(unsigned) + 2 * sizeof(float4) +
-                           (size_t)n_on * sizeof(unsigned);
+    const size_t per_ray = per_ray_bytes(n_on);
     size_t need = 0;
     pl.no_sync = true;
     for (int k = 0; k <= max_level; ++k) {
@@ -633,8 +649,7 @@ static int host_buffers(Workspace &ws) {
 }
 
 int launch_wavefront(const SceneRes &S, const rtg_params &p, int W, int H, const RenderOut &o, hipStream_t stream,
-                     Arena &A) {
-    Workspace &ws = A.ws;
+                     Workspace &ws, Arena &A, const int *cancel_host) {
     double dp;
     int st = upload_raygen_tables(S, W, H, stream, ws, dp);
     if (st != RTG_OK) return st;
@@ -654,8 +669,12 @@ int launch_wavefront(const SceneRes &S, const rtg_params &p, int W, int H, const
     a.n_samples = r.n_samples;
     a.counts = o.counts;
     WavePlan pl = make_plan(r.n_samples, p.max_level, S.max_fanout, __builtin_popcount(S.lights_on), 1);
+    // A large frame keeps the GPU busy at every level; there, sizing each
+    // level from the previous level's count beats launching full grids
+    // against the worst-case bounds.
+    if (r.n_samples > kNoSyncMaxSamples) pl.no_sync = false;
     if (pl.no_sync && (st = reserve_all(ws, pl, A.cus > 0 ? A.cus : 256)) != RTG_OK) return st;
-    return enqueue_wave<false>(S.view, a, pl, stream, ws, A, A.cancel_host);
+    return enqueue_wave<false>(S.view, a, pl, stream, ws, A, cancel_host);
 }
 
 // Pinned staging for batch tables; replaced only while `stream` is idle.
@@ -672,7 +691,7 @@ static int staging(Workspace &ws, size_t bytes, hipStream_t stream) {
 }
 
 int launch_batch(const BatchItem *items, int n, unsigned long long *counts, hipStream_t stream, Workspace &ws,
-                 Arena &A, bool use_graph, int *graph_replayed) {
+                 Arena &A, int graph_policy, int *graph_replayed) {
     if (graph_replayed) *graph_replayed = 0;
     if (n <= 0) return RTG_OK;
     if (n > kMaxBatch) return fail(RTG_E_LIMIT, "batch larger than 4096 requests");
@@ -722,7 +741,7 @@ int launch_batch(const BatchItem *items, int n, unsigned long long *counts, hipS
         r.sample_base = (int)padded;
     }
     WavePlan pl = make_plan((int)padded, max_level, fanout, n_on, n_pad);
-    if (!pl.no_sync) use_graph = false;
+    bool use_graph = pl.no_sync && graph_policy != GRAPH_NEVER;
     int st = host_buffers(ws);
     if (st != RTG_OK) return st;
     if (pl.no_sync && (st = reserve_all(ws, pl, A.cus > 0 ? A.cus : 256)) != RTG_OK) return st;
@@ -768,6 +787,14 @@ int launch_batch(const BatchItem *items, int n, unsigned long long *counts, hipS
                                                                       (int)n_chunks);
         return enqueue_wave<true>(SceneView{}, a, pl, stream, ws, A, nullptr);
     };
+    if (use_graph && graph_policy == GRAPH_SECOND && !(ws.graph != nullptr && ws.graph_key == key)) {
+        bool seen = false;
+        for (uint64_t s : ws.seen_keys) seen = seen || s == key;
+        if (!seen) {
+            ws.seen_keys[ws.seen_next++ & 7] = key;
+            use_graph = false;
+        }
+    }
     if (!use_graph) return enqueue();
     if (ws.graph == nullptr || ws.graph_key != key) {
         if (ws.graph) (void)hipGraphExecDestroy(ws.graph);

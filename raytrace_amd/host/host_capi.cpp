@@ -55,6 +55,12 @@ int rtg_host_scene_config(int config, const rtg_gen_opts *opts, rtg_host_scene_t
     return RTG_OK;
 }
 
+int rtg_host_gen_requests(uint64_t seed, int n, double rate, rtg_gen_request *out) {
+    if (n < 0 || (n > 0 && out == nullptr) || !(rate >= 0)) return RTG_E_INVALID;
+    rtg_make_requests(seed, n, rate, out);
+    return RTG_OK;
+}
+
 int rtg_host_scene_new(rtg_host_scene_t *out) {
     if (out == nullptr) return RTG_E_INVALID;
     auto *h = new rtg_host_scene_s();

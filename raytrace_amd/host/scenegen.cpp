@@ -423,6 +423,22 @@ static Scene *gen_config5(const rtg_gen_opts &o, rtg_gen_info &info) {
     return s;
 }
 
+// Request stream of workloads 3/4: scene U{0..7}, 64 round(16^u) x
+// 64 round(12^u) pixels, Poisson arrivals at `rate` (exponential gaps).
+void rtg_make_requests(uint64_t seed, int n, double rate, rtg_gen_request *out) {
+    SplitMix64 rng(mix64(seed ? seed : 3));
+    double t = 0.0;
+    for (int k = 0; k < n; ++k) {
+        rtg_gen_request &q = out[k];
+        q.scene = std::min(7, (int)(rng.u() * 8.0));
+        q.width = 64 * (int)std::llround(std::pow(16.0, rng.u()));
+        q.height = 64 * (int)std::llround(std::pow(12.0, rng.u()));
+        q.pad = 0;
+        if (rate > 0) t += -std::log(1.0 - rng.u()) / rate;
+        q.arrival_s = rate > 0 ? t : 0.0;
+    }
+}
+
 Scene *rtg_make_config_scene(int config, const rtg_gen_opts &opts, rtg_gen_info &info) {
     Scene *s = nullptr;
     switch (config) {
