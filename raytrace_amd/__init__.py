@@ -223,7 +223,8 @@ def gen_requests(n: int, rate: float = 0.0, seed: int = 3):
     return [(buf[i].scene, buf[i].width, buf[i].height, buf[i].arrival_s) for i in range(n)]
 
 
-def serve(items, mode: int = _capi.RTG_SERVE_BATCHED, slots: int = 0, max_batch: int = 0, hbm_budget: int = 0):
+def serve(items, mode: int = _capi.RTG_SERVE_BATCHED, slots: int = 0, max_batch: int = 0, hbm_budget: int = 0,
+          graph_after: int = 0):
     """rtg_serve over [(DeviceScene, rtg_params, arrival_s), ...].
     Returns ([rgb per request], [(done_s, batch) per request], rtg_serve_stats)."""
     lib = _capi.load()
@@ -237,7 +238,8 @@ def serve(items, mode: int = _capi.RTG_SERVE_BATCHED, slots: int = 0, max_batch:
         reqs[k].rgb = rgb.ctypes.data_as(C.POINTER(C.c_uint8))
         reqs[k].arrival_s = t
         outs.append(rgb)
-    o = _capi.rtg_serve_opts(mode=mode, slots=slots, max_batch=max_batch, hbm_budget=hbm_budget)
+    o = _capi.rtg_serve_opts(mode=mode, slots=slots, max_batch=max_batch, hbm_budget=hbm_budget,
+                             graph_after=graph_after)
     st = _capi.rtg_serve_stats()
     check(lib.rtg_serve(reqs, n, C.byref(o), C.byref(st)), "rtg_serve")
     return outs, [(reqs[k].done_s, reqs[k].batch) for k in range(n)], st

@@ -113,7 +113,7 @@ class rtg_serve_request(C.Structure):
 
 
 class rtg_serve_opts(C.Structure):
-    _fields_ = [("mode", C.c_int32), ("slots", C.c_int32), ("max_batch", C.c_int32), ("pad", C.c_int32),
+    _fields_ = [("mode", C.c_int32), ("slots", C.c_int32), ("max_batch", C.c_int32), ("graph_after", C.c_int32),
                 ("hbm_budget", C.c_uint64)]
 
 

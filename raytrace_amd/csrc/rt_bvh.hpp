@@ -65,7 +65,7 @@ RTG_HD Box6 padded_box(const double lo[3], const double hi[3]) {
 // `pos` the model position, `mid` the part's model-space border midpoint
 // ((borders[2p] + borders[2p+1]) * 0.5, Model.cpp:420).
 RTG_HD void prim_setup_tri(const rtg_triangle &tri, V3 pos, V3 mid, int g, int obj, float4 *prim3, float4 *nrm3,
-                           float2 *uv3, uint8_t &bins, Box6 &box) {
+                           float2 *uv3, float4 *pos3, uint8_t &bins, Box6 &box) {
     const V3 p0 = V3{tri.p[0].x, tri.p[0].y, tri.p[0].z}, p1 = V3{tri.p[1].x, tri.p[1].y, tri.p[1].z},
              p2 = V3{tri.p[2].x, tri.p[2].y, tri.p[2].z};
     const V3 e1 = p1 - p0, e2 = p2 - p0, pw = p0 + pos;
@@ -75,6 +75,7 @@ RTG_HD void prim_setup_tri(const rtg_triangle &tri, V3 pos, V3 mid, int g, int o
     for (int k = 0; k < 3; ++k) {
         nrm3[k] = make_float4(tri.n[k].x, tri.n[k].y, tri.n[k].z, 0.0f);
         uv3[k] = make_float2(tri.uv[2 * k], tri.uv[2 * k + 1]);
+        pos3[k] = make_float4(tri.p[k].x, tri.p[k].y, tri.p[k].z, 0.0f);
     }
     // octant bins by the model-space triangle bounds (_mm_min_ps/_mm_max_ps)
     float mn[3], mx[3];

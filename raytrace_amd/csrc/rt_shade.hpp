@@ -238,6 +238,7 @@ RTG_HD V3 compose(V3 c, float rl, float rr, bool tir, bool inside, V3 kd, V3 c_r
 
 struct RayCounts {
     uint32_t primary, shadow, reflect, refract;
+    int retests;  // closest-hit queries re-tested in FP64
 };
 
 struct PrimaryOut {
@@ -258,7 +259,7 @@ RTG_HD int id_prim(const SceneView &S, const Hit &h) {
 // One sample of the megakernel: the ray tree of RTfrac evaluated depth first
 // with an explicit frame stack; children are composed into their parent in
 // the reference's order.
-template <class Stack>
+template <bool kRetest = false, class Stack>
 RTG_HD V3 render_sample(const SceneView &S, int max_level, const TraceRay &primary, Stack &st, RayCounts &rc,
                         PrimaryOut &po) {
     struct Frame {
@@ -288,7 +289,7 @@ RTG_HD V3 render_sample(const SceneView &S, int max_level, const TraceRay &prima
             else
                 rc.refract++;
             if (q.src.obj >= 0) q.self_test = self_testable(S, q.r, q.src, kMiss);
-            const Hit h = trace_closest(S, q, st);
+            const Hit h = trace_closest<kRetest>(S, q, st, &rc.retests);
             if (q_level == 0) po = PrimaryOut{h.obj, id_prim(S, h), h.t};
             const float zNear = q_level == 0 ? S.cam.zNear : 0.0f;
             if (h.kind == HIT_NONE || h.t > zFar || h.t < zNear) {

@@ -205,13 +205,112 @@ RTG_HD bool brute_objects(const SceneView &S, const TraceRay &q, bool anyhit, fl
     return false;
 }
 
-// Closest hit (RayTracer.cpp:455-465).
-template <class Stack>
-RTG_HD Hit trace_closest(const SceneView &S, const TraceRay &q, Stack &st) {
+// ---- FP64 near-tie re-test (RTG_FLAG_FP64_RETEST) --------------------------
+//
+// A closest-hit query can be decided by FP32 rounding: two candidates within
+// 4 ulp of each other, a barycentric or t within 1e-6 of its accept threshold,
+// a ray tangent to a sphere.  With the re-test, the walk collects such
+// candidates (the same band the oracle flags) and, when there are any,
+// re-tests them and the FP32 winner in FP64 from the exact model-space
+// vertices (object position added in double); the FP64 closest wins, ties to
+// the lower (object, primitive).  Hits outside the band keep their FP32
+// result, so frames differ from the reference algorithm only inside the band.
+
+struct D3 {
+    double x, y, z;
+};
+RTG_HD D3 d3(V3 a) { return D3{a.x, a.y, a.z}; }
+RTG_HD D3 dsub(D3 a, D3 b) { return D3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+RTG_HD D3 dadd(D3 a, D3 b) { return D3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+RTG_HD double ddot(D3 a, D3 b) { return (a.x * b.x + a.y * b.y) + (a.z * b.z + 0.0); }
+RTG_HD D3 dcross(D3 a, D3 b) { return D3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+
+// tri_test in double (same steps and accept rules).
+RTG_HD double tri_test64(D3 o, D3 d, D3 p0, D3 e1, D3 e2, double &w, double &u, double &v) {
+    const D3 tmp1 = dcross(d, e2), t2r = dsub(o, p0);
+    const double f = 1.0 / ddot(e1, tmp1);
+    u = ddot(t2r, tmp1) * f;
+    if (u < 0.0 || u > 1.0) return 1e20;
+    const D3 tmp2 = dcross(t2r, e1);
+    v = ddot(d, tmp2) * f;
+    w = 1.0 - u - v;
+    if (v < 0.0 || w < 0.0) return 1e20;
+    const double t = ddot(e2, tmp2) * f;
+    return t > 1e-5 ? t : 1e20;
+}
+
+// sphere_enter_t / sphere_exit_t in double.
+RTG_HD double sphere_t64(D3 o, D3 d, D3 c, double rsq, bool exit) {
+    const D3 s2r = dsub(o, c);
+    const double b = ddot(d, s2r);
+    if (!exit && b > 0) return 1e20;
+    const double dis = b * b - ddot(s2r, s2r) + rsq;
+    if (!exit && dis < 0) return 1e20;
+    return exit ? -b + sqrt(dis) : -(b + sqrt(dis));
+}
+
+struct NearBand {
+    static constexpr int kMax = 6;
+    int n = 0;
+    int pi[kMax];
+    float t[kMax];
+    RTG_HD void add(int p, float tt) {
+        for (int k = 0; k < n; ++k)
+            if (pi[k] == p) return;
+        if (n < kMax) pi[n] = p, t[n] = tt, ++n;
+    }
+};
+
+// FP64 re-test of primitive slot pi for query q; fills `h` when it hits.
+RTG_HD bool retest64(const SceneView &S, const TraceRay &q, int pi, double &t64, Hit &h) {
+    const D3 o = d3(q.r.o), d = d3(q.r.d);
+    const float4 A = S.prims[3 * pi];
+    const int tag = as_int(A.w);
+    if (tag >= 0) {
+        const float4 B = S.prims[3 * pi + 1];
+        const int obj = as_int(B.w);
+        if (tag == q.src.prim && obj == q.src.obj && !q.self_test) return false;
+        const float4 *P = S.tri_pos + 3 * (size_t)tag;
+        const D3 v0 = D3{P[0].x, P[0].y, P[0].z}, v1 = D3{P[1].x, P[1].y, P[1].z}, v2 = D3{P[2].x, P[2].y, P[2].z};
+        const D3 p0 = dadd(v0, d3(S.objs[obj].pos));
+        double w, u, v;
+        t64 = tri_test64(o, d, p0, dsub(v1, v0), dsub(v2, v0), w, u, v);
+        if (!(t64 < 1e19)) return false;
+        h = Hit{(float)t64, HIT_TRI, obj, tag, (float)w, (float)u, (float)v};
+        return true;
+    }
+    const int obj = ~tag;
+    const bool exit = q.src.obj == obj;
+    if (exit && !q.inside) return false;
+    const DevObject &ob = S.objs[obj];
+    t64 = sphere_t64(o, d, d3(ob.pos), (double)ob.radius * (double)ob.radius, exit);
+    if (!(t64 > 1e-6 && t64 < 1e19)) return false;
+    h = Hit{(float)t64, HIT_SPHERE, obj, -1, 0, 0, 0};
+    return true;
+}
+
+// Closest hit (RayTracer.cpp:455-465).  kRetest: FP64 near-tie re-test (see
+// above); *retested counts the queries that were re-tested.
+template <bool kRetest = false, class Stack>
+RTG_HD Hit trace_closest(const SceneView &S, const TraceRay &q, Stack &st, int *retested = nullptr) {
     Hit best{kMiss, HIT_NONE, -1, -1, 0, 0, 0};
     brute_objects(S, q, false, 0.0f, best);
     uint64_t bkey = best.kind != HIT_NONE ? hit_key(best.obj, best.prim) : ~0ull;
     float tmax = best.t;
+    int bpi = -1, pi2 = -1;  // primitive slots of the FP32 winner and runner-up (BVH primitives)
+    float t2 = kMiss;
+    NearBand band;
+    auto take = [&](float t, uint64_t k, int pi, const Hit &h, float &tm) {
+        if (t < best.t || (t == best.t && k < bkey)) {
+            if (kRetest && bpi >= 0) t2 = best.t, pi2 = bpi;
+            best = h;
+            bkey = k;
+            bpi = pi;
+            tm = t;
+        } else if (kRetest && t < t2) {
+            t2 = t, pi2 = pi;
+        }
+    };
     bvh_walk(S, q.r, tmax, st, [&](int pi, float &tm) -> bool {
         const float4 A = S.prims[3 * pi];
         const int tag = as_int(A.w);
@@ -220,16 +319,30 @@ RTG_HD Hit trace_closest(const SceneView &S, const TraceRay &q, Stack &st) {
             const float4 B = S.prims[3 * pi + 1], C = S.prims[3 * pi + 2];
             const int obj = as_int(B.w);
             if (tag == q.src.prim && obj == q.src.obj && !q.self_test) return false;
-            float w, u, v;
-            const float t = tri_test(q.r, V3{B.x, B.y, B.z}, V3{C.x, C.y, C.z}, p, w, u, v);
-            if (t < kMiss) {
-                const uint64_t k = hit_key(obj, tag);
-                if (t < best.t || (t == best.t && k < bkey)) {
-                    best = Hit{t, HIT_TRI, obj, tag, w, u, v};
-                    bkey = k;
-                    tm = t;
+            const V3 e1 = V3{B.x, B.y, B.z}, e2 = V3{C.x, C.y, C.z};
+            float w, u, v, t;
+            if constexpr (kRetest) {
+                // tri_test's values without its early exits (same operations)
+                const V3 tmp1 = cross(q.r.d, e2), t2r = q.r.o - p;
+                const float f = 1.0f / dot(e1, tmp1);
+                u = dot(t2r, tmp1) * f;
+                const V3 tmp2 = cross(t2r, e1);
+                v = dot(q.r.d, tmp2) * f;
+                w = 1.0f - u - v;
+                const float tt = dot(e2, tmp2) * f;
+                if (tt == tt && tt > 0.0f) {
+                    const float eps = 1e-6f;
+                    const bool marginal = fabsf(u) <= eps || fabsf(u - 1.0f) <= eps || fabsf(v) <= eps ||
+                                          fabsf(w) <= eps || fabsf(tt - 1e-5f) <= eps;
+                    const bool inside = u >= -eps && u <= 1.0f + eps && v >= -eps && w >= -eps;
+                    if (marginal && inside) band.add(pi, tt);
                 }
+                const bool ok = !(u < 0.0f || u > 1.0f) && !(v < 0.0f || w < 0.0f) && tt > 1e-5f;
+                t = ok ? tt : kMiss;
+            } else {
+                t = tri_test(q.r, e1, e2, p, w, u, v);
             }
+            if (t < kMiss) take(t, hit_key(obj, tag), pi, Hit{t, HIT_TRI, obj, tag, w, u, v}, tm);
         } else {  // sphere, tag = ~object
             const int obj = ~tag;
             const float4 B = S.prims[3 * pi + 1];
@@ -240,17 +353,41 @@ RTG_HD Hit trace_closest(const SceneView &S, const TraceRay &q, Stack &st) {
             } else {
                 t = sphere_enter_t(q.r, p, B.x);
             }
-            if (t > 1e-6f && t < kMiss) {
-                const uint64_t k = hit_key(obj, -1);
-                if (t < best.t || (t == best.t && k < bkey)) {
-                    best = Hit{t, HIT_SPHERE, obj, -1, 0, 0, 0};
-                    bkey = k;
-                    tm = t;
-                }
+            if constexpr (kRetest) {
+                const V3 s2r = q.r.o - p;
+                const float b = dot(q.r.d, s2r);
+                const float dis = b * b - dot(s2r, s2r) + B.x;
+                if (fabsf(dis) <= 1e-5f * (B.x + b * b)) band.add(pi, -b);
             }
+            if (t > 1e-6f && t < kMiss) take(t, hit_key(obj, -1), pi, Hit{t, HIT_SPHERE, obj, -1, 0, 0, 0}, tm);
         }
         return false;
     });
+    if constexpr (kRetest) {
+        if (bpi >= 0 || best.kind == HIT_NONE) {
+            const bool tie = bpi >= 0 && pi2 >= 0 && t2 < kMiss && t2 - best.t <= 4.0f * (nextafterf(best.t, kMiss) - best.t);
+            const float lim = best.kind == HIT_NONE ? 1e30f : best.t * (1.0f + 1e-5f);
+            int cand[NearBand::kMax + 2], nc = 0;
+            for (int k = 0; k < band.n; ++k)
+                if (band.t[k] <= lim) cand[nc++] = band.pi[k];
+            if (tie || nc > 0) {
+                if (bpi >= 0) cand[nc++] = bpi;
+                if (tie) cand[nc++] = pi2;
+                double bt = 1e300;
+                uint64_t bk = ~0ull;
+                Hit nb = best;
+                for (int k = 0; k < nc; ++k) {
+                    double t64;
+                    Hit h;
+                    if (!retest64(S, q, cand[k], t64, h)) continue;
+                    const uint64_t key = hit_key(h.obj, h.prim);
+                    if (t64 < bt || (t64 == bt && key < bk)) bt = t64, bk = key, nb = h;
+                }
+                if (bt < 1e19 && (nb.obj != best.obj || nb.prim != best.prim || nb.kind != best.kind)) best = nb;
+                if (retested) ++*retested;
+            }
+        }
+    }
     return best;
 }
 

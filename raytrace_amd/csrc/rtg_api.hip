@@ -1,4 +1,5 @@
 // rtg_api.hip — implementation of the librtg C ABI (include/rtg.h).
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -186,11 +187,15 @@ static int render_device(SceneRes &S, const rtg_params &p, int W, int H, const R
     k_fill_u8<<<1024, 256, 0, stream>>>(o.rgb, npix * 3, 127);
     if (o.ids) k_fill_u8<<<1024, 256, 0, stream>>>(reinterpret_cast<uint8_t *>(o.ids), npix * 8, 0xff);
     k_zero_u64<<<1, 64, 0, stream>>>(o.counts, 4);
+    if (o.retests) k_zero_u64<<<1, 64, 0, stream>>>(o.retests, 1);
     if (p.mode != RTG_MODE_RAYTRACE && p.mode != RTG_MODE_REFRACTTEST && p.mode != 0)
         return fail(RTG_E_INVALID, "render mode not supported by this build");
-    // AUTO: the wavefront engine (faster on every measured workload; the
-    // megakernel's per-thread ray trees diverge on deep refraction trees).
+    // AUTO: the wavefront engine.  The megakernel is slower on every large or
+    // deep frame (its per-thread ray trees diverge); it only wins on small,
+    // shallow frames that take well under a millisecond either way.
     int engine = p.engine == RTG_ENGINE_AUTO ? RTG_ENGINE_WAVEFRONT : p.engine;
+    if (engine == RTG_ENGINE_MEGAKERNEL && (p.flags & RTG_FLAG_FP64_RETEST))
+        return fail(RTG_E_INVALID, "the FP64 near-tie re-test needs the wavefront engine");
     int st = A.ws.order_before(stream);
     if (st != RTG_OK) return st;
     st = engine == RTG_ENGINE_WAVEFRONT ? launch_wavefront(S, p, W, H, o, stream, A.ws, A, A.cancel_host)
@@ -229,13 +234,15 @@ int rtg_init(int device) {
     hipDeviceProp_t prop;
     RTG_HIP(hipGetDeviceProperties(&prop, device));
     A->cus = prop.multiProcessorCount;
+    A->no_sync_budget = std::min<size_t>(size_t(64) << 30, std::max<size_t>(size_t(2) << 30, prop.totalGlobalMem / 4));
     RTG_HIP(hipStreamCreateWithFlags(&A->stream, hipStreamNonBlocking));
     RTG_HIP(hipEventCreate(&A->ev0));
     RTG_HIP(hipEventCreate(&A->ev1));
     RTG_HIP(hipHostMalloc(reinterpret_cast<void **>(&A->cancel_host), sizeof(int), hipHostMallocMapped));
     *A->cancel_host = 0;
     RTG_HIP(hipHostGetDevicePointer(reinterpret_cast<void **>(&A->cancel_dev), A->cancel_host, 0));
-    if (ensure(A->counters, 4 * sizeof(unsigned long long))) return fail(RTG_E_NOMEM, "counters");
+    // ray counts [0..3] (primary, shadow, reflection, refraction), FP64 re-tests [4]
+    if (ensure(A->counters, 8 * sizeof(unsigned long long))) return fail(RTG_E_NOMEM, "counters");
     g_arena = std::move(A);
     return RTG_OK;
 }
@@ -307,6 +314,7 @@ int rtg_scene_update(rtg_scene_t scene, const rtg_scene_desc *desc) {
     std::swap(R.parts.p, fresh.parts.p), std::swap(R.parts.bytes, fresh.parts.bytes);
     std::swap(R.tri_nrm.p, fresh.tri_nrm.p), std::swap(R.tri_nrm.bytes, fresh.tri_nrm.bytes);
     std::swap(R.tri_uv.p, fresh.tri_uv.p), std::swap(R.tri_uv.bytes, fresh.tri_uv.bytes);
+    std::swap(R.tri_pos.p, fresh.tri_pos.p), std::swap(R.tri_pos.bytes, fresh.tri_pos.bytes);
     std::swap(R.tri_part.p, fresh.tri_part.p), std::swap(R.tri_part.bytes, fresh.tri_part.bytes);
     std::swap(R.tri_bins.p, fresh.tri_bins.p), std::swap(R.tri_bins.bytes, fresh.tri_bins.bytes);
     std::swap(R.mtls.p, fresh.mtls.p), std::swap(R.mtls.bytes, fresh.mtls.bytes);
@@ -364,15 +372,15 @@ static int render_host(rtg_scene_t scene, const rtg_params *params, uint8_t *rgb
         *A.cancel_host = 0;
     }
     RenderOut o{A.out_rgb.as<uint8_t>(), ids ? A.out_ids.as<int32_t>() : nullptr, tprim ? A.out_t.as<float>() : nullptr,
-                A.counters.as<unsigned long long>()};
+                A.counters.as<unsigned long long>(), A.counters.as<unsigned long long>() + 4};
     RTG_HIP(hipEventRecord(A.ev0, A.stream));
     st = render_device(scene->res, *params, W, H, o, A.stream, A);
     RTG_HIP(hipEventRecord(A.ev1, A.stream));
-    unsigned long long cnt[4] = {0, 0, 0, 0};
+    unsigned long long cnt[5] = {0, 0, 0, 0, 0};
     RTG_HIP(hipMemcpyAsync(rgb, o.rgb, npix * 3, hipMemcpyDeviceToHost, A.stream));
     if (ids) RTG_HIP(hipMemcpyAsync(ids, o.ids, npix * 8, hipMemcpyDeviceToHost, A.stream));
     if (tprim) RTG_HIP(hipMemcpyAsync(tprim, o.tprim, npix * 4, hipMemcpyDeviceToHost, A.stream));
-    RTG_HIP(hipMemcpyAsync(cnt, o.counts, sizeof(cnt), hipMemcpyDeviceToHost, A.stream));
+    RTG_HIP(hipMemcpyAsync(cnt, o.counts, sizeof(cnt), hipMemcpyDeviceToHost, A.stream));  // counts + re-tests
     RTG_HIP(hipStreamSynchronize(A.stream));
     if (job) job->running = 0;
     if (st != RTG_OK) return st;
@@ -388,7 +396,7 @@ static int render_host(rtg_scene_t scene, const rtg_params *params, uint8_t *rgb
         stats->rays_total = cnt[0] + cnt[1] + cnt[2] + cnt[3];
         stats->render_ms = ms;
         stats->build_ms = scene->res.build_ms;
-        stats->retests_fp64 = 0;
+        stats->retests_fp64 = cnt[4];
     }
     if (cancelled) return fail(RTG_E_CANCELLED, "render cancelled");
     return RTG_OK;
@@ -411,7 +419,7 @@ int rtg_render_device(rtg_scene_t scene, const rtg_params *params, uint8_t *rgb_
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : A.stream;
     unsigned long long *counts =
         stats_dev ? reinterpret_cast<unsigned long long *>(stats_dev) : A.counters.as<unsigned long long>();
-    RenderOut o{rgb_dev, ids_dev, tprim_dev, counts};
+    RenderOut o{rgb_dev, ids_dev, tprim_dev, counts, A.counters.as<unsigned long long>() + 4};
     return render_device(scene->res, *params, W, H, o, s, A);
 }
 
@@ -527,14 +535,14 @@ int rtg_render_batch(const rtg_request *reqs, int n, rtg_stats *stats) {
     unsigned long long *counts = A.counters.as<unsigned long long>();
     k_fill_u8<<<1024, 256, 0, s>>>(A.out_rgb.as<uint8_t>(), rgb_bytes, 127);
     if (ids_bytes) k_fill_u8<<<1024, 256, 0, s>>>(A.out_ids.as<uint8_t>(), ids_bytes, 0xff);
-    k_zero_u64<<<1, 64, 0, s>>>(counts, 4);
+    k_zero_u64<<<1, 64, 0, s>>>(counts, 5);
     RTG_HIP(hipEventRecord(A.ev0, s));
     int replayed = 0;
     int st = A.ws.order_before(s);
-    if (st == RTG_OK) st = launch_batch(items.data(), n, counts, s, A.ws, A, eager ? GRAPH_NEVER : GRAPH_FIRST, &replayed);
+    if (st == RTG_OK) st = launch_batch(items.data(), n, counts, counts + 4, s, A.ws, A, eager ? 0 : 1, &replayed);
     if (st == RTG_OK) st = A.ws.mark_done(s);
     RTG_HIP(hipEventRecord(A.ev1, s));
-    unsigned long long cnt[4] = {0, 0, 0, 0};
+    unsigned long long cnt[5] = {0, 0, 0, 0, 0};
     if (st == RTG_OK) {
         for (int i = 0; i < n; ++i) {
             const size_t npix = (size_t)items[i].W * items[i].H;
@@ -555,6 +563,7 @@ int rtg_render_batch(const rtg_request *reqs, int n, rtg_stats *stats) {
         stats->rays_total = cnt[0] + cnt[1] + cnt[2] + cnt[3];
         stats->render_ms = ms;
         stats->graph_replays = (uint64_t)replayed;
+        stats->retests_fp64 = cnt[4];
     }
     return RTG_OK;
 }

@@ -17,14 +17,14 @@ constexpr int kBlock = 256;
 
 __global__ void k_setup_tris(const rtg_triangle *raw, int n, const int *tri_part, const DevPart *parts,
                              const float4 *part_mid, const DevObject *objs, float4 *prims, Box6 *boxes,
-                             float4 *nrm, float2 *uv, uint8_t *bins) {
+                             float4 *nrm, float2 *uv, float4 *pos, uint8_t *bins) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     const int pi = tri_part[g];
     const DevPart &P = parts[pi];
     const float4 m = part_mid[pi];
     prim_setup_tri(raw[g], objs[P.obj].pos, V3{m.x, m.y, m.z}, g, P.obj, prims + 3 * (size_t)g, nrm + 3 * (size_t)g,
-                   uv + 3 * (size_t)g, bins[g], boxes[g]);
+                   uv + 3 * (size_t)g, pos + 3 * (size_t)g, bins[g], boxes[g]);
 }
 
 __global__ void k_setup_spheres(const int *sph, int n, int base, const DevObject *objs, float4 *prims, Box6 *boxes) {
@@ -213,6 +213,7 @@ int scene_build(SceneRes &S, const rtg_scene_desc &d, hipStream_t stream) {
     }
     const size_t nt = std::max<int64_t>(n_tris, 1);
     if (ensure(S.tri_nrm, nt * 3 * sizeof(float4)) || ensure(S.tri_uv, nt * 3 * sizeof(float2)) ||
+        ensure(S.tri_pos, nt * 3 * sizeof(float4)) ||
         ensure(S.tri_bins, nt) || ensure(prims_u, std::max(n, 1) * 3 * sizeof(float4)) ||
         ensure(boxes, std::max(n, 1) * sizeof(Box6)))
         return fail(RTG_E_NOMEM, "device allocation failed (scene)");
@@ -225,7 +226,8 @@ int scene_build(SceneRes &S, const rtg_scene_desc &d, hipStream_t stream) {
         k_setup_tris<<<nblk(n_tris), kBlock, 0, stream>>>(raw.as<rtg_triangle>(), (int)n_tris, S.tri_part.as<int>(),
                                                           S.parts.as<DevPart>(), mid.as<float4>(), S.objs.as<DevObject>(),
                                                           prims_u.as<float4>(), boxes.as<Box6>(), S.tri_nrm.as<float4>(),
-                                                          S.tri_uv.as<float2>(), S.tri_bins.as<uint8_t>());
+                                                          S.tri_uv.as<float2>(), S.tri_pos.as<float4>(),
+                                                          S.tri_bins.as<uint8_t>());
     if (!spheres.empty())
         k_setup_spheres<<<nblk(spheres.size()), kBlock, 0, stream>>>(sph.as<int>(), (int)spheres.size(), (int)n_tris,
                                                                      S.objs.as<DevObject>(), prims_u.as<float4>(),
@@ -327,6 +329,7 @@ int scene_build(SceneRes &S, const rtg_scene_desc &d, hipStream_t stream) {
     v.parts = S.parts.as<DevPart>();
     v.tri_nrm = S.tri_nrm.as<float4>();
     v.tri_uv = S.tri_uv.as<float2>();
+    v.tri_pos = S.tri_pos.as<float4>();
     v.tri_part = S.tri_part.as<int>();
     v.tri_bins = S.tri_bins.as<uint8_t>();
     v.mtls = S.mtls.as<DevMaterial>();
@@ -339,7 +342,7 @@ int scene_build(SceneRes &S, const rtg_scene_desc &d, hipStream_t stream) {
     v.cam = prep.cam;
     S.camera = d.camera;
     S.bytes = S.nodes.bytes + S.prims.bytes + S.objs.bytes + S.brute.bytes + S.parts.bytes + S.tri_nrm.bytes +
-              S.tri_uv.bytes + S.tri_part.bytes + S.tri_bins.bytes + S.mtls.bytes + S.lights.bytes + S.texs.bytes +
+              S.tri_uv.bytes + S.tri_pos.bytes + S.tri_part.bytes + S.tri_bins.bytes + S.mtls.bytes + S.lights.bytes + S.texs.bytes +
               S.texels.bytes;
     return RTG_OK;
 }

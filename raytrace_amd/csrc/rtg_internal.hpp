@@ -39,7 +39,7 @@ struct DevBuf {
 
 struct SceneRes {
     int device = 0;
-    DevBuf nodes, prims, objs, brute, parts, tri_nrm, tri_uv, tri_part, tri_bins, mtls, lights, texs, texels;
+    DevBuf nodes, prims, objs, brute, parts, tri_nrm, tri_uv, tri_pos, tri_part, tri_bins, mtls, lights, texs, texels;
     SceneView view{};
     rtg_camera camera{};
     uint64_t geom_hash = 0;
@@ -67,7 +67,9 @@ struct Workspace {
     uint64_t epoch = 0;  // bumped when a buffer moves (captured graphs hold addresses)
     hipGraphExec_t graph = nullptr;
     uint64_t graph_key = 0;
-    uint64_t seen_keys[8] = {};  // batch shapes launched recently (GRAPH_SECOND)
+    static constexpr int kSeen = 32;
+    uint64_t seen_keys[kSeen] = {};  // batch shapes launched recently, with launch counts
+    uint32_t seen_count[kSeen] = {};
     int seen_next = 0;
     // Recorded after the last render enqueued with this workspace: a render
     // on another stream waits for it (device-side) before reusing the buffers.
@@ -105,6 +107,9 @@ struct Arena {
     static constexpr int kLevels = Workspace::kLevels;
     int device = -1;
     int cus = 0;
+    // Largest queue memory (bytes) one wavefront may reserve to run without
+    // host round trips between levels: a quarter of HBM, 2..64 GiB.
+    size_t no_sync_budget = size_t(8) << 30;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBuf counters, out_rgb, out_ids, out_t;
@@ -155,6 +160,7 @@ struct RenderOut {
     int32_t *ids;   // device, optional
     float *tprim;   // device, optional
     unsigned long long *counts;  // device, 4 counters (accumulated)
+    unsigned long long *retests = nullptr;  // device, 1 counter: FP64 re-tests (optional)
 };
 
 // Launch one frame of the megakernel on `stream` (asynchronous).
@@ -177,17 +183,15 @@ struct BatchItem {
 };
 // Enqueue every request of a batch on `stream` as one wavefront: all their
 // rays share one queue per level; no host round trip when the queue bounds fit
-// the workspace budget.  The launch sequence is captured into a hipGraph and
-// replayed for later batches of the same padded shape: GRAPH_NEVER,
-// GRAPH_FIRST (capture at the first launch of a shape) or GRAPH_SECOND
-// (capture when a shape comes back; one-off shapes run eagerly).
+// the workspace budget.  With capture_at = k > 0 the launch sequence of a
+// padded batch shape is captured into a hipGraph at its k-th launch and
+// replayed afterwards (shapes seen fewer times run eagerly); 0: never.
 // `counts` (device, 4 x u64) accumulates the ray counts of the whole batch.
-enum GraphPolicy { GRAPH_NEVER = 0, GRAPH_FIRST = 1, GRAPH_SECOND = 2 };
-int launch_batch(const BatchItem *items, int n, unsigned long long *counts, hipStream_t stream, Workspace &ws,
-                 Arena &A, int graph_policy, int *graph_replayed);
+int launch_batch(const BatchItem *items, int n, unsigned long long *counts, unsigned long long *retests,
+                 hipStream_t stream, Workspace &ws, Arena &A, int capture_at, int *graph_replayed);
 // Device bytes the queues of a batch reserve (its no-sync bound), for
-// admission by memory budget; 0 when the bound exceeds the no-sync budget.
-size_t batch_queue_bytes(size_t samples, int max_level, int fanout, int n_on);
+// admission by memory budget; 0 when the bound exceeds `budget`.
+size_t batch_queue_bytes(size_t samples, int max_level, int fanout, int n_on, size_t budget);
 // Level-0 queue length of a single frame.
 inline size_t frame_samples(int W, int H, int spp) { return (size_t)((W + 7) / 8) * ((H + 7) / 8) * 64 * (spp > 0 ? spp : 1); }
 

@@ -50,7 +50,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_megakernel(SceneView S, MegaA
     __shared__ int lds[kLdsStack * kBlockThreads];
     const int tid = threadIdx.x;
     DevStack st = make_stack(lds, A.spill);
-    RayCounts rc{0, 0, 0, 0};
+    RayCounts rc{0, 0, 0, 0, 0};
     const ChunkRange cr = xcd_chunks(A.nblocks);
     const int tw = (A.W >> 6) << 6, th = (A.H >> 6) << 6;
     for (int pb = cr.first; pb < cr.end; pb += cr.step) {

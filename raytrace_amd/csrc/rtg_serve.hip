@@ -10,11 +10,13 @@
 // batch's RGB to pinned memory and an event.  Several slots run at once, so
 // a batch admitted while others are on the GPU overlaps them on the device.
 // A completed slot's RGB is copied into each request's buffer and the
-// request is done.  Shapes that come back are replayed from captured graphs.
+// request is done.  A batch shape launched `graph_after` times is captured
+// into a hipGraph and replayed from then on.
 //
 // Plain mode.  One slot, one request at a time, each rendered as its own
 // frame (the single-frame wavefront), in arrival order: the plain renderer
-// the loop is compared with.
+// the loop is compared with.  (A batched-mode batch of one request takes the
+// same single-frame path.)
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -31,7 +33,7 @@ __global__ void k_fill_bytes(uint8_t *p, size_t n, uint8_t v) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
 __global__ void k_zero_counts(unsigned long long *p) {
-    if (threadIdx.x < 4) p[threadIdx.x] = 0;
+    if (threadIdx.x < 5) p[threadIdx.x] = 0;  // 4 ray counts + FP64 re-tests
 }
 
 using Clock = std::chrono::steady_clock;
@@ -44,7 +46,7 @@ int slot_init(ServeSlot &s) {
     if (s.counts_host == nullptr)
         RTG_HIP(hipHostMalloc(reinterpret_cast<void **>(&s.counts_host), 4 * sizeof(unsigned long long),
                               hipHostMallocDefault));
-    if (ensure(s.counts, 4 * sizeof(unsigned long long))) return fail(RTG_E_NOMEM, "serve counters");
+    if (ensure(s.counts, 5 * sizeof(unsigned long long))) return fail(RTG_E_NOMEM, "serve counters");
     return RTG_OK;
 }
 
@@ -81,7 +83,7 @@ ServeSlot::~ServeSlot() {
 // Enqueue `ids` as one batch on slot `s` (asynchronous; `s.done` marks the
 // end of the RGB copy to pinned memory).
 static int slot_launch(ServeSlot &s, rtg_serve_request *reqs, const std::vector<ReqInfo> &info,
-                       const std::vector<int> &ids, bool plain, Arena &A) {
+                       const std::vector<int> &ids, bool plain, int capture_at, Arena &A) {
     size_t bytes = 0;
     s.offs.resize(ids.size());
     for (size_t k = 0; k < ids.size(); ++k) {
@@ -92,9 +94,10 @@ static int slot_launch(ServeSlot &s, rtg_serve_request *reqs, const std::vector<
     if (st != RTG_OK) return st;
     k_fill_bytes<<<1024, 256, 0, s.stream>>>(s.out.as<uint8_t>(), bytes, 127);
     k_zero_counts<<<1, 64, 0, s.stream>>>(s.counts.as<unsigned long long>());
-    if (plain) {
+    if (plain || ids.size() == 1) {  // a batch of one takes the single-frame path (uniform kernel arguments)
         const rtg_serve_request &q = reqs[ids[0]];
-        const RenderOut o{s.out.as<uint8_t>(), nullptr, nullptr, s.counts.as<unsigned long long>()};
+        const RenderOut o{s.out.as<uint8_t>(), nullptr, nullptr, s.counts.as<unsigned long long>(),
+                          s.counts.as<unsigned long long>() + 4};
         st = launch_wavefront(q.scene->res, q.params, info[ids[0]].W, info[ids[0]].H, o, s.stream, s.ws, A, nullptr);
     } else {
         std::vector<BatchItem> items(ids.size());
@@ -104,8 +107,8 @@ static int slot_launch(ServeSlot &s, rtg_serve_request *reqs, const std::vector<
                                  s.out.as<uint8_t>() + s.offs[k], nullptr, nullptr};
         }
         int replayed = 0;
-        st = launch_batch(items.data(), (int)items.size(), s.counts.as<unsigned long long>(), s.stream, s.ws, A,
-                          GRAPH_SECOND, &replayed);
+        st = launch_batch(items.data(), (int)items.size(), s.counts.as<unsigned long long>(),
+                          s.counts.as<unsigned long long>() + 4, s.stream, s.ws, A, capture_at, &replayed);
     }
     if (st != RTG_OK) return st;
     RTG_HIP(hipMemcpyAsync(s.pinned, s.out.p, bytes, hipMemcpyDeviceToHost, s.stream));
@@ -122,6 +125,7 @@ int serve_run(rtg_serve_request *reqs, int n, const rtg_serve_opts &o, rtg_serve
     const int nslots = plain ? 1 : (o.slots > 0 ? std::min(o.slots, 8) : 2);
     const int max_batch = plain ? 1 : (o.max_batch > 0 ? std::min(o.max_batch, 4096) : 256);
     const size_t budget = o.hbm_budget > 0 ? (size_t)o.hbm_budget : (size_t(4) << 30);
+    const int capture_at = o.graph_after < 0 ? 0 : (o.graph_after == 0 ? 3 : o.graph_after);
     std::vector<ReqInfo> info(n);
     for (int i = 0; i < n; ++i) {
         rtg_serve_request &q = reqs[i];
@@ -193,13 +197,13 @@ int serve_run(rtg_serve_request *reqs, int n, const rtg_serve_opts &o, rtg_serve
                 const ReqInfo &r = info[id];
                 const int l2 = std::max(lvl, (int)reqs[id].params.max_level), f2 = std::max(fan, r.fanout),
                           o2 = std::max(on, r.n_on);
-                const size_t need = batch_queue_bytes(batch_bucket(samples + r.samples), l2, f2, o2);
+                const size_t need = batch_queue_bytes(batch_bucket(samples + r.samples), l2, f2, o2, A.no_sync_budget);
                 if (!ids.empty() && (need == 0 || need > budget)) break;  // the next batch takes it
                 ids.push_back(id);
                 samples += r.samples, lvl = l2, fan = f2, on = o2;
                 ++head;
             }
-            st = slot_launch(s, reqs, info, ids, plain, A);
+            st = slot_launch(s, reqs, info, ids, plain, capture_at, A);
             if (st != RTG_OK) break;
             for (int id : ids) reqs[id].batch = batches;
             ++batches;

@@ -64,6 +64,7 @@ constexpr int kCtrWords = (int)(sizeof(LevelCtr) / sizeof(unsigned));
 // of the batch's request table (B = true).
 struct WaveReq {
     int W, H, spp, max_level, tile_compat;
+    int retest;       // RTG_FLAG_FP64_RETEST: FP64 near-tie re-test of closest hits
     int bw;           // 8x8 pixel blocks per row
     int n_samples;    // pixel blocks * 64 * spp
     int sample_base;  // first level-0 queue slot of the request
@@ -83,6 +84,7 @@ struct WaveArgs {
     const int *chunk_req;     // B = true: request of each 64-sample chunk of level 0 (-1: padding)
     int n_samples;            // level-0 queue length
     unsigned long long *counts;
+    unsigned long long *retests;  // closest-hit queries re-tested in FP64
     int *spill;
 };
 
@@ -187,7 +189,9 @@ __device__ inline int sample_req(const WaveArgs &A, int i) {
         return 0;
 }
 
-template <bool B>
+// R: some request asks for the FP64 near-tie re-test (a separate kernel, so
+// the plain closest-hit walk keeps its registers).
+template <bool B, bool R>
 __global__ __launch_bounds__(kBlockThreads, 6) void k_extend(SceneView S0, WaveArgs A, int k, const WRay *rays,
                                                              const unsigned *n_ptr, WHit *hits, LineCtr *sched) {
     __shared__ int lds[kLdsStack * kBlockThreads];
@@ -195,6 +199,7 @@ __global__ __launch_bounds__(kBlockThreads, 6) void k_extend(SceneView S0, WaveA
     const int n = k == 0 ? A.n_samples : (int)*n_ptr;
     WaveQueue wq(sched, n);
     uint32_t c_prim = 0, c_refl = 0, c_refr = 0;
+    int n_retest = 0;
     for (int base = wq.next(); base >= 0; base = wq.next()) {
         const int i = base + __lane_id();
         if (i >= n) continue;
@@ -206,14 +211,14 @@ __global__ __launch_bounds__(kBlockThreads, 6) void k_extend(SceneView S0, WaveA
             w = rays[i];
             r = w.flags >> 8;
         }
-        const WaveReq &R = req_at<B>(A, r < 0 ? 0 : r);
-        const SceneView &S = scene_at<B>(S0, A, R.scene);
+        const WaveReq &Rq = req_at<B>(A, r < 0 ? 0 : r);
+        const SceneView &S = scene_at<B>(S0, A, Rq.scene);
         TraceRay q;
         int px = 0, py = 0, si = 0;
         bool valid = r >= 0;
         if (k == 0) {
-            if (valid) valid = sample_pixel(R, i - R.sample_base, px, py, si);
-            if (valid) q = primary_ray_w(S, R, px, py, si);
+            if (valid) valid = sample_pixel(Rq, i - Rq.sample_base, px, py, si);
+            if (valid) q = primary_ray_w(S, Rq, px, py, si);
         } else {
             float bwc, mtlrfr;
             q = load_ray(w, bwc, mtlrfr);
@@ -221,7 +226,10 @@ __global__ __launch_bounds__(kBlockThreads, 6) void k_extend(SceneView S0, WaveA
         }
         Hit h{kMiss, HIT_NONE, -1, -1, 0, 0, 0};
         if (valid) {
-            h = trace_closest(S, q, st);
+            if constexpr (R)
+                h = Rq.retest ? trace_closest<true>(S, q, st, &n_retest) : trace_closest<false>(S, q, st);
+            else
+                h = trace_closest(S, q, st);
             if (k == 0)
                 ++c_prim;
             else if (q.type == RAY_REFLECT)
@@ -231,19 +239,23 @@ __global__ __launch_bounds__(kBlockThreads, 6) void k_extend(SceneView S0, WaveA
         }
         hits[i] = WHit{h.t, h.w, h.u, h.v, h.kind, h.obj, h.prim, 0};
         if (k == 0 && valid && si == 0) {
-            const size_t pix = (size_t)py * R.W + px;
-            if (R.ids) {
-                R.ids[2 * pix] = h.obj;
-                R.ids[2 * pix + 1] = id_prim(S, h);
+            const size_t pix = (size_t)py * Rq.W + px;
+            if (Rq.ids) {
+                Rq.ids[2 * pix] = h.obj;
+                Rq.ids[2 * pix + 1] = id_prim(S, h);
             }
-            if (R.tprim) R.tprim[pix] = h.t;
+            if (Rq.tprim) Rq.tprim[pix] = h.t;
         }
     }
     add_counts(A.counts, c_prim, 0, c_refl, c_refr);
+    if constexpr (R) {
+        const uint32_t nr = wave_sum_u32((uint32_t)n_retest);
+        if (__lane_id() == 0 && nr) atomicAdd(A.retests, (unsigned long long)nr);
+    }
 }
 
 template <bool B>
-__global__ __launch_bounds__(kBlockThreads) void k_shade(SceneView S0, WaveArgs A, int k, const WRay *rays,
+__global__ __launch_bounds__(kBlockThreads, 4) void k_shade(SceneView S0, WaveArgs A, int k, const WRay *rays,
                                                          const unsigned *n_ptr, const WHit *hits, WNode *nodes,
                                                          unsigned *masks, WRay *next, LevelCtr *ctr,
                                                          unsigned *shadow) {
@@ -467,9 +479,6 @@ __global__ void k_chunk_req(const WaveReq *reqs, int n_reqs, int *chunk_req, int
     chunk_req[c] = r;
 }
 
-// Largest queue memory (bytes) a frame may reserve to run without host round
-// trips between levels.
-constexpr size_t kNoSyncBudget = size_t(8) << 30;
 // Single frames with more samples run level by level (see launch_wavefront).
 constexpr int kNoSyncMaxSamples = 1 << 20;
 
@@ -486,7 +495,7 @@ template <bool B>
 struct Grids {
     int extend, shade, occlude;
     explicit Grids(int cus) {
-        static const int oe = blocks_per_cu(k_extend<B>), os = blocks_per_cu(k_shade<B>),
+        static const int oe = blocks_per_cu(k_extend<B, false>), os = blocks_per_cu(k_shade<B>),
                          oo = blocks_per_cu(k_occlude<B>);
         extend = cus * oe, shade = cus * os, occlude = cus * oo;
     }
@@ -508,17 +517,17 @@ static size_t per_ray_bytes(int n_on) {
            (size_t)n_on * sizeof(unsigned);
 }
 
-size_t batch_queue_bytes(size_t samples, int max_level, int fanout, int n_on) {
+size_t batch_queue_bytes(size_t samples, int max_level, int fanout, int n_on, size_t budget) {
     size_t need = 0, b = samples;
     for (int k = 0; k <= max_level && b > 0; ++k) {
         need += b * per_ray_bytes(n_on);
-        if (need > kNoSyncBudget) return 0;
+        if (need > budget) return 0;
         b *= (size_t)fanout;
     }
     return need;
 }
 
-static WavePlan make_plan(int n_samples, int max_level, int fanout, int n_on, int n_reqs) {
+static WavePlan make_plan(int n_samples, int max_level, int fanout, int n_on, int n_reqs, size_t budget) {
     WavePlan pl{};
     pl.n_samples = n_samples, pl.max_level = max_level, pl.fanout = fanout, pl.n_on = n_on, pl.n_reqs = n_reqs;
     const size_t per_ray = per_ray_bytes(n_on);
@@ -527,7 +536,7 @@ static WavePlan make_plan(int n_samples, int max_level, int fanout, int n_on, in
     for (int k = 0; k <= max_level; ++k) {
         pl.bound[k] = k == 0 ? (size_t)n_samples : std::min<size_t>(pl.bound[k - 1] * (size_t)fanout, size_t(1) << 40);
         need += pl.bound[k] * per_ray;
-        if (need > kNoSyncBudget) pl.no_sync = false;
+        if (need > budget) pl.no_sync = false;
     }
     return pl;
 }
@@ -565,7 +574,7 @@ static int reserve_all(Workspace &ws, const WavePlan &pl, int cus) {
 // allocates or waits.
 template <bool B>
 static int enqueue_wave(const SceneView &S0, const WaveArgs &a, const WavePlan &pl, hipStream_t stream,
-                        Workspace &ws, Arena &A, const int *cancel_host) {
+                        Workspace &ws, Arena &A, const int *cancel_host, bool retest) {
     const int cus = A.cus > 0 ? A.cus : 256;
     const Grids<B> g(cus);
     if (!pl.no_sync) {
@@ -594,8 +603,12 @@ static int enqueue_wave(const SceneView &S0, const WaveArgs &a, const WavePlan &
         const unsigned *n_ptr = k == 0 ? nullptr : &ctr[k - 1].n_next.v;
         WRay *next = ws.w_rays[k + 1 < Workspace::kLevels ? k + 1 : k].as<WRay>();
         int kt = ktime_begin(A, stream, KT_EXTEND, n_k);
-        k_extend<B><<<grid(g.extend, n_k), kBlockThreads, 0, stream>>>(S0, args, k, rays, n_ptr,
-                                                                       ws.w_hits[k].as<WHit>(), ctr[k].sched[0]);
+        if (retest)
+            k_extend<B, true><<<grid(g.extend, n_k), kBlockThreads, 0, stream>>>(
+                S0, args, k, rays, n_ptr, ws.w_hits[k].as<WHit>(), ctr[k].sched[0]);
+        else
+            k_extend<B, false><<<grid(g.extend, n_k), kBlockThreads, 0, stream>>>(
+                S0, args, k, rays, n_ptr, ws.w_hits[k].as<WHit>(), ctr[k].sched[0]);
         ktime_end(A, stream, kt, n_ptr);
         kt = ktime_begin(A, stream, KT_SHADE, n_k);
         k_shade<B><<<grid(g.shade, n_k), kBlockThreads, 0, stream>>>(
@@ -663,18 +676,21 @@ int launch_wavefront(const SceneRes &S, const rtg_params &p, int W, int H, const
     r.scene = 0;
     r.dpf = (float)dp;
     r.seed = p.seed;
+    r.retest = (p.flags & RTG_FLAG_FP64_RETEST) != 0;
     r.ax = ws.tables.as<float>();
     r.by = ws.tables.as<float>() + W;
     r.rgb = o.rgb, r.ids = o.ids, r.tprim = o.tprim;
     a.n_samples = r.n_samples;
     a.counts = o.counts;
-    WavePlan pl = make_plan(r.n_samples, p.max_level, S.max_fanout, __builtin_popcount(S.lights_on), 1);
+    a.retests = o.retests;
+    WavePlan pl = make_plan(r.n_samples, p.max_level, S.max_fanout, __builtin_popcount(S.lights_on), 1,
+                            A.no_sync_budget);
     // A large frame keeps the GPU busy at every level; there, sizing each
     // level from the previous level's count beats launching full grids
     // against the worst-case bounds.
     if (r.n_samples > kNoSyncMaxSamples) pl.no_sync = false;
     if (pl.no_sync && (st = reserve_all(ws, pl, A.cus > 0 ? A.cus : 256)) != RTG_OK) return st;
-    return enqueue_wave<false>(S.view, a, pl, stream, ws, A, cancel_host);
+    return enqueue_wave<false>(S.view, a, pl, stream, ws, A, cancel_host, r.retest != 0);
 }
 
 // Pinned staging for batch tables; replaced only while `stream` is idle.
@@ -690,8 +706,8 @@ static int staging(Workspace &ws, size_t bytes, hipStream_t stream) {
     return RTG_OK;
 }
 
-int launch_batch(const BatchItem *items, int n, unsigned long long *counts, hipStream_t stream, Workspace &ws,
-                 Arena &A, int graph_policy, int *graph_replayed) {
+int launch_batch(const BatchItem *items, int n, unsigned long long *counts, unsigned long long *retests,
+                 hipStream_t stream, Workspace &ws, Arena &A, int capture_at, int *graph_replayed) {
     if (graph_replayed) *graph_replayed = 0;
     if (n <= 0) return RTG_OK;
     if (n > kMaxBatch) return fail(RTG_E_LIMIT, "batch larger than 4096 requests");
@@ -716,6 +732,7 @@ int launch_batch(const BatchItem *items, int n, unsigned long long *counts, hipS
     while (n_pad < n) n_pad <<= 1;
     std::vector<WaveReq> reqs(n_pad);
     size_t total = 0;
+    bool any_retest = false;
     for (int i = 0; i < n; ++i) {
         const BatchItem &it = items[i];
         WaveReq &r = reqs[i];
@@ -726,6 +743,8 @@ int launch_batch(const BatchItem *items, int n, unsigned long long *counts, hipS
         r.sample_base = (int)total;
         r.scene = scene_of[i];
         r.seed = it.params.seed;
+        r.retest = (it.params.flags & RTG_FLAG_FP64_RETEST) != 0;
+        any_retest = any_retest || r.retest;
         r.rgb = it.rgb, r.ids = it.ids, r.tprim = it.tprim;
         total += (size_t)r.n_samples;
     }
@@ -740,8 +759,8 @@ int launch_batch(const BatchItem *items, int n, unsigned long long *counts, hipS
         r.spp = 1;
         r.sample_base = (int)padded;
     }
-    WavePlan pl = make_plan((int)padded, max_level, fanout, n_on, n_pad);
-    bool use_graph = pl.no_sync && graph_policy != GRAPH_NEVER;
+    WavePlan pl = make_plan((int)padded, max_level, fanout, n_on, n_pad, A.no_sync_budget);
+    bool use_graph = pl.no_sync && capture_at > 0;
     int st = host_buffers(ws);
     if (st != RTG_OK) return st;
     if (pl.no_sync && (st = reserve_all(ws, pl, A.cus > 0 ? A.cus : 256)) != RTG_OK) return st;
@@ -779,21 +798,26 @@ int launch_batch(const BatchItem *items, int n, unsigned long long *counts, hipS
     a.chunk_req = ws.b_chunks.as<int>();
     a.n_samples = (int)padded;
     a.counts = counts;
+    a.retests = retests;
     const uint64_t key = (uint64_t)padded ^ ((uint64_t)n_pad << 32) ^ ((uint64_t)max_level << 45) ^
-                         ((uint64_t)fanout << 50) ^ ((uint64_t)n_on << 53) ^ (ws.epoch * 0x9E3779B97F4A7C15ull) ^
-                         ((uint64_t)(uintptr_t)counts * 0xC2B2AE3D27D4EB4Full);
+                         ((uint64_t)fanout << 50) ^ ((uint64_t)n_on << 53) ^ ((uint64_t)any_retest << 60) ^
+                         (ws.epoch * 0x9E3779B97F4A7C15ull) ^ ((uint64_t)(uintptr_t)counts * 0xC2B2AE3D27D4EB4Full) ^
+                         ((uint64_t)(uintptr_t)retests * 0x165667B19E3779F9ull);
     auto enqueue = [&]() -> int {
         k_chunk_req<<<(int)((n_chunks + 255) / 256), 256, 0, stream>>>(a.reqs, n_pad, ws.b_chunks.as<int>(),
                                                                       (int)n_chunks);
-        return enqueue_wave<true>(SceneView{}, a, pl, stream, ws, A, nullptr);
+        return enqueue_wave<true>(SceneView{}, a, pl, stream, ws, A, nullptr, any_retest);
     };
-    if (use_graph && graph_policy == GRAPH_SECOND && !(ws.graph != nullptr && ws.graph_key == key)) {
-        bool seen = false;
-        for (uint64_t s : ws.seen_keys) seen = seen || s == key;
-        if (!seen) {
-            ws.seen_keys[ws.seen_next++ & 7] = key;
-            use_graph = false;
+    if (use_graph && capture_at > 1 && !(ws.graph != nullptr && ws.graph_key == key)) {
+        int slot = -1;
+        for (int i = 0; i < Workspace::kSeen; ++i)
+            if (ws.seen_count[i] && ws.seen_keys[i] == key) slot = i;
+        if (slot < 0) {
+            slot = ws.seen_next++ % Workspace::kSeen;
+            ws.seen_keys[slot] = key;
+            ws.seen_count[slot] = 0;
         }
+        if ((int)++ws.seen_count[slot] < capture_at) use_graph = false;
     }
     if (!use_graph) return enqueue();
     if (ws.graph == nullptr || ws.graph_key != key) {

@@ -47,14 +47,14 @@ def emul_render(desc_ptr: int, p, threads: int = 8, tiles=None) -> dict:
     rgb = np.full((H, W, 3), 127, np.uint8)
     ids = np.full((H, W, 2), -1, np.int32)
     t = np.full((H, W), np.float32(1e20), np.float32)
-    cnt = np.zeros(4, np.uint64)
+    cnt = np.zeros(5, np.uint64)  # ray counts by type, FP64 re-tests
     depth = C.c_int()
     tl = np.ascontiguousarray(np.asarray(tiles, np.int32)) if tiles is not None else None
     ptr = lambda a: a.ctypes.data_as(C.c_void_p) if a is not None else None
     rc = emul_lib().emul_render(C.c_void_p(desc_ptr), C.byref(p), threads, ptr(tl), 0 if tl is None else len(tl),
                                 ptr(rgb), ptr(ids), ptr(t), ptr(cnt), C.byref(depth))
     assert rc == 0
-    return {"rgb": rgb, "ids": ids, "t": t, "counts": cnt, "depth": depth.value,
+    return {"rgb": rgb, "ids": ids, "t": t, "counts": cnt[:4], "retests": int(cnt[4]), "depth": depth.value,
             "max_stack": emul_lib().emul_max_stack()}
 
 
@@ -101,3 +101,19 @@ def rt_gpu():
     import raytrace_amd as rt
     rt.init(0)
     return rt
+
+
+def fp64_stats(test: dict, o64: dict, region=None) -> dict:
+    """Pixel-error statistics of a frame against the FP64 oracle: primary-id
+    mismatch fraction, RGB mean / max absolute error (in 1/255 steps) and the
+    fraction of pixels off by more than one step."""
+    sel = np.ones(o64["rgb"].shape[:2], bool) if region is None else region
+    ids_eq = (test["ids"] == o64["ids"]).all(-1)
+    d = np.abs(test["rgb"].astype(np.int16) - o64["rgb"].astype(np.int16))
+    return {
+        "n": int(sel.sum()),
+        "ids_mismatch": float((~ids_eq & sel).sum() / max(1, sel.sum())),
+        "rgb_mae": float(d[sel].mean()) if sel.any() else 0.0,
+        "rgb_max": int(d[sel].max()) if sel.any() else 0,
+        "rgb_over_1": float(((d.max(-1) > 1) & sel).sum() / max(1, sel.sum())),
+    }

@@ -43,6 +43,7 @@ struct HostScene {
     std::vector<BvhNode> nodes;
     std::vector<float4> prims, tri_nrm;
     std::vector<float2> tri_uv;
+    std::vector<float4> tri_pos;
     std::vector<uint8_t> tri_bins;
     SceneView view{};
     int depth = 0;
@@ -57,6 +58,7 @@ int build(const rtg_scene_desc &d, HostScene &H) {
     std::vector<Box6> boxes(std::max(n, 1));
     H.tri_nrm.resize((size_t)std::max<int64_t>(nt, 1) * 3);
     H.tri_uv.resize((size_t)std::max<int64_t>(nt, 1) * 3);
+    H.tri_pos.resize((size_t)std::max<int64_t>(nt, 1) * 3);
     H.tri_bins.resize((size_t)std::max<int64_t>(nt, 1));
     int64_t g = 0;
     for (const int pi : P.part_order) {
@@ -64,7 +66,7 @@ int build(const rtg_scene_desc &d, HostScene &H) {
         const float4 m = P.part_mid[pi];
         for (int t = 0; t < d.parts[pi].tri_count; ++t, ++g)
             prim_setup_tri(P.part_tris[pi][t], P.objs[DP.obj].pos, V3{m.x, m.y, m.z}, (int)g, DP.obj, &prims_u[3 * g],
-                           &H.tri_nrm[3 * g], &H.tri_uv[3 * g], H.tri_bins[g], boxes[g]);
+                           &H.tri_nrm[3 * g], &H.tri_uv[3 * g], &H.tri_pos[3 * g], H.tri_bins[g], boxes[g]);
     }
     for (size_t k = 0; k < P.spheres.size(); ++k) {
         const DevObject &o = P.objs[P.spheres[k]];
@@ -141,6 +143,7 @@ int build(const rtg_scene_desc &d, HostScene &H) {
     v.parts = P.parts.data();
     v.tri_nrm = H.tri_nrm.data();
     v.tri_uv = H.tri_uv.data();
+    v.tri_pos = H.tri_pos.data();
     v.tri_part = P.tri_part.data();
     v.tri_bins = H.tri_bins.data();
     v.mtls = P.mtls.data();
@@ -172,11 +175,12 @@ int emul_render(const rtg_scene_desc *d, const rtg_params *p, int threads, const
     const int tx = (W + 63) / 64, ty = (Ht + 63) / 64;
     const int total = tiles ? n_tiles : tx * ty;
     std::atomic<int> next{0};
-    std::atomic<uint64_t> c0{0}, c1{0}, c2{0}, c3{0};
+    std::atomic<uint64_t> c0{0}, c1{0}, c2{0}, c3{0}, c4{0};
+    const bool retest = (p->flags & RTG_FLAG_FP64_RETEST) != 0;
     g_max_sp = 0;
     auto work = [&]() {
         TrackStack st;
-        RayCounts rc{0, 0, 0, 0};
+        RayCounts rc{0, 0, 0, 0, 0};
         for (;;) {
             const int k = next.fetch_add(1);
             if (k >= total) break;
@@ -207,7 +211,8 @@ int emul_render(const rtg_scene_desc *d, const rtg_params *p, int threads, const
                         q.src = no_src();
                         q.self_test = 0;
                         PrimaryOut p0;
-                        const V3 c = render_sample(H.view, p->max_level, q, st, rc, p0);
+                        const V3 c = retest ? render_sample<true>(H.view, p->max_level, q, st, rc, p0)
+                                            : render_sample<false>(H.view, p->max_level, q, st, rc, p0);
                         if (s == 0) po = p0;
                         acc = acc + c;
                     }
@@ -220,12 +225,12 @@ int emul_render(const rtg_scene_desc *d, const rtg_params *p, int threads, const
                     if (tprim) tprim[pix] = po.t;
                 }
         }
-        c0 += rc.primary, c1 += rc.shadow, c2 += rc.reflect, c3 += rc.refract;
+        c0 += rc.primary, c1 += rc.shadow, c2 += rc.reflect, c3 += rc.refract, c4 += (uint64_t)rc.retests;
     };
     std::vector<std::thread> th;
     for (int i = 0; i < std::max(1, threads); ++i) th.emplace_back(work);
     for (auto &t : th) t.join();
-    if (counts) counts[0] = c0, counts[1] = c1, counts[2] = c2, counts[3] = c3;
+    if (counts) counts[0] = c0, counts[1] = c1, counts[2] = c2, counts[3] = c3, counts[4] = c4;
     return 0;
 }
 

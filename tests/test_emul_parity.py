@@ -108,3 +108,27 @@ def test_batch_bucket_is_whole_chunks():
         assert p <= max(4096, t + t // 8 + 4096), (t, p)
     # batches of similar size share a bucket (and so a captured graph)
     assert f(2325760) == f(2325760 + 64)
+
+
+@pytest.mark.parametrize("workload", [1, 2])
+def test_fp64_retest_changes_only_band_pixels(workload):
+    """RTG_FLAG_FP64_RETEST (device code on the CPU): near-tie closest hits are
+    decided in FP64.  Against the reference algorithm, only pixels inside the
+    oracle's near-tie band may change; against the FP64 oracle, primary ids
+    must not get worse."""
+    from oracle import pyoracle as po
+    from conftest import fp64_stats
+    kw = {} if workload == 1 else dict(width=480, height=270)
+    pr = dict(width=640, height=480, max_level=3) if workload == 1 else dict(width=480, height=270, max_level=4)
+    hs = rt.HostScene.workload(workload, **kw)
+    p = rt.params(**pr)
+    pf = rt.params(**pr, flags=rt.RTG_FLAG_FP64_RETEST)
+    osc = po.OracleScene(hs.desc_ptr)
+    o32, o64 = osc.render(p, po.ORACLE_REF32), osc.render(p, po.ORACLE_FP64)
+    e, ef = emul_render(hs.desc_ptr, p), emul_render(hs.desc_ptr, pf)
+    s = compare_frames(ef, o32)
+    assert s["ids_mismatch_outside_band"] == 0 and s["rgb_over_1"] == 0, s
+    assert ef["retests"] > 0
+    changed = ~(ef["ids"] == e["ids"]).all(-1)
+    assert ((o32["flags"][changed] & 3) != 0).all()  # every change is a band pixel
+    assert fp64_stats(ef, o64)["ids_mismatch"] <= fp64_stats(e, o64)["ids_mismatch"]

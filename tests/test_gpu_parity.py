@@ -189,3 +189,47 @@ def test_workload5_reduced_vs_oracle(rt_gpu):
     s = compare_frames(g, o, region=tile_mask(960, 540, tiles))
     log(f"  {s}  rays {g['stats'].rays_total}")
     _check(s)
+
+
+def test_fp64_retest_on_device(rt_gpu):
+    """RTG_FLAG_FP64_RETEST on the GPU: identical to the device code run on the
+    CPU with the flag, and different from the reference algorithm only inside
+    the oracle's near-tie band."""
+    from conftest import emul_render
+    rt = rt_gpu
+    log("case: workload 2 at 480x270 with the FP64 near-tie re-test")
+    hs = rt.HostScene.workload(2, width=480, height=270)
+    pf = rt.params(480, 270, max_level=4, flags=rt.RTG_FLAG_FP64_RETEST)
+    g = rt.DeviceScene(hs).render(pf)
+    e = emul_render(hs.desc_ptr, pf)
+    assert (g["rgb"] == e["rgb"]).all() and (g["ids"] == e["ids"]).all() and (g["t"] == e["t"]).all()
+    assert g["stats"].retests_fp64 == e["retests"] > 0
+    s = compare_frames(g, _oracle(hs, rt.params(480, 270, max_level=4)))
+    log(f"  {s}  re-tests {g['stats'].retests_fp64}")
+    assert s["ids_mismatch_outside_band"] == 0 and s["rgb_over_1"] == 0
+
+
+@pytest.mark.parametrize("workload", [1, 2, 3, 5])
+def test_fp64_error_statistics(rt_gpu, workload):
+    """Pixel-error statistics of the device frames (reference FP32 semantics,
+    and with the FP64 near-tie re-test) against the FP64 oracle."""
+    from conftest import fp64_stats
+    from oracle import pyoracle as po
+    rt = rt_gpu
+    if workload == 1:
+        hs, pr = rt.HostScene.workload(1), dict(width=640, height=480, max_level=3)
+    elif workload == 2:
+        hs, pr = rt.HostScene.workload(2, width=480, height=270), dict(width=480, height=270, max_level=4)
+    elif workload == 3:
+        hs = rt.HostScene.workload(3, index=1, width=320, height=256)
+        pr = dict(width=320, height=256, max_level=hs.info.max_level)
+    else:
+        hs = rt.HostScene.workload(5, width=320, height=180, detail=8, n_spheres=64, n_lights=16)
+        pr = dict(width=320, height=180, max_level=8, spp=4, seed=5)
+    ds = rt.DeviceScene(hs)
+    g = ds.render(rt.params(**pr))
+    gf = ds.render(rt.params(**pr, flags=rt.RTG_FLAG_FP64_RETEST))
+    o64 = po.OracleScene(hs.desc_ptr).render(rt.params(**pr), po.ORACLE_FP64)
+    a, b = fp64_stats(g, o64), fp64_stats(gf, o64)
+    log(f"case: workload {workload} vs FP64 oracle: fp32 {a}; fp32+fp64 re-test {b}, re-tests {gf['stats'].retests_fp64}")
+    assert a["ids_mismatch"] < 0.01 and b["ids_mismatch"] <= a["ids_mismatch"] + 1e-4
