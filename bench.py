@@ -145,6 +145,73 @@ def bench_serve(args) -> dict:
     }
 
 
+def bench_stress(args) -> dict:
+    """Workload 5: 3840x2160, 4 jittered samples, depth 8, refraction; a
+    4,188,162-triangle terrain + 256 sphere meshes (2,048 triangles each),
+    16 lights.  Frames timed on the device; pixel errors against the FP64
+    oracle and parity against the reference algorithm on tile samples."""
+    import numpy as np
+    import raytrace_amd as rt
+    from oracle import pyoracle as po
+    rt.init(0)
+    log = lambda m: (sys.stderr.write(m + "\n"), sys.stderr.flush())
+    hs = rt.HostScene.workload(5)
+    W, H = hs.info.width, hs.info.height
+    log(f"workload 5: {hs.info.n_tris} triangles, {W}x{H}, building")
+    ds = rt.DeviceScene(hs)
+    p = rt.params(W, H, max_level=8, spp=4, seed=5)
+    frames = []
+    for k in range(max(1, args.warmup) + max(1, args.steps)):
+        t0 = time.perf_counter()
+        out = ds.render(p)
+        frames.append((time.perf_counter() - t0, out["stats"]))
+        log(f"frame {k}: {out['stats'].rays_total} rays, device {out['stats'].render_ms:.1f} ms")
+    timed = frames[max(1, args.warmup):]
+    dev_ms = sum(st.render_ms for _, st in timed) / len(timed)
+    st = timed[-1][1]
+    osc = po.OracleScene(hs.desc_ptr)
+    tx, ty = (W + 63) // 64, (H + 63) // 64
+    rng = np.random.default_rng(5)
+    tiles = np.sort(rng.choice(tx * ty, size=(tx * ty) // 64, replace=False)).astype(np.int32)
+    threads = max(1, min(64, os.cpu_count() or 1))
+    log(f"FP64 oracle on {len(tiles)} tiles")
+    o64 = osc.render(p, po.ORACLE_FP64, threads=threads, tiles=tiles)
+    m = np.zeros((H, W), bool)
+    for t in tiles:
+        bx, by = t % tx, t // tx
+        m[by * 64:by * 64 + 64, bx * 64:bx * 64 + 64] = True
+    ids_eq = (out["ids"] == o64["ids"]).all(-1)
+    d = np.abs(out["rgb"].astype(np.int16) - o64["rgb"].astype(np.int16))
+    fp64 = {"pixels": int(m.sum()), "ids_mismatch": round(float((~ids_eq & m).sum() / m.sum()), 6),
+            "rgb_mae": round(float(d[m].mean()), 4), "rgb_max": int(d[m].max()),
+            "rgb_over_1": round(float(((d.max(-1) > 1) & m).sum() / m.sum()), 6)}
+    ref_tiles = tiles[:: max(1, len(tiles) // 3)][:3]
+    log(f"reference-algorithm oracle on {len(ref_tiles)} tiles")
+    o32 = osc.render(p, po.ORACLE_REF32, threads=threads, tiles=ref_tiles)
+    mr = np.zeros((H, W), bool)
+    for t in ref_tiles:
+        bx, by = t % tx, t // tx
+        mr[by * 64:by * 64 + 64, bx * 64:bx * 64 + 64] = True
+    eq = (out["ids"] == o32["ids"]).all(-1)
+    band = (o32["flags"] & 3) != 0
+    dr = np.abs(out["rgb"].astype(np.int16) - o32["rgb"].astype(np.int16)).max(-1)
+    parity = {"pixels": int(mr.sum()), "ids_match": round(float(eq[mr].mean()), 6),
+              "ids_mismatch_outside_band": int((~eq & ~band & mr).sum()), "rgb_over_1": int(((dr > 1) & eq & mr).sum())}
+    return {
+        "metric": "Mrays/s (primary+secondary), 4K stress frame", "value": round(st.rays_total / dev_ms / 1e3, 2),
+        "unit": "Mrays/s", "n_gpus": 1, "steps": len(timed), "warmup": len(frames) - len(timed),
+        "ms_per_step": round(sum(t for t, _ in timed) / len(timed) * 1e3, 1), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "3840x2160, 4 spp jittered, depth 8, refraction IOR 1.3-1.7, 4,188,162-tri terrain + "
+                               "256 x 2,048-tri sphere meshes, 16 lights", "device_ms_per_frame": round(dev_ms, 2),
+                   "rays_per_frame": st.rays_total,
+                   "rays_by_type": {"primary": st.rays_primary, "shadow": st.rays_shadow, "reflect": st.rays_reflect,
+                                    "refract": st.rays_refract},
+                   "bvh_build_ms": round(ds.build_ms, 2), "scene_bytes": ds.nbytes, "triangles": int(hs.info.n_tris)},
+        "fp64_error": fp64, "parity_vs_reference_algorithm": parity,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,8 +220,9 @@ def main() -> int:
     ap.add_argument("--engine", default="auto", choices=["mega", "wave", "auto"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-tiles", type=int, default=32, help="1/N of the 64x64 tiles for the CPU baseline sample")
-    ap.add_argument("--workload", type=int, default=2, choices=[2, 3, 4],
-                    help="2: the 1080p frame (default line); 3: 256 batched requests; 4: serving sweep")
+    ap.add_argument("--workload", type=int, default=2, choices=[2, 3, 4, 5],
+                    help="2: the 1080p frame (default line); 3: 256 batched requests; 4: serving sweep; "
+                         "5: 4K stress frame")
     ap.add_argument("--rates", default="50,100,200,500,1000,2000", help="workload 4: offered rates (requests/s)")
     ap.add_argument("--duration", type=float, default=2.0, help="workload 4: seconds of arrivals per rate")
     ap.add_argument("--slots", type=int, default=2, help="workload 4: concurrent batches")
@@ -179,6 +247,9 @@ def main() -> int:
         return 0
     if args.workload == 4:
         print(json.dumps(bench_serve(args)))
+        return 0
+    if args.workload == 5:
+        print(json.dumps(bench_stress(args)))
         return 0
 
     import numpy as np

@@ -66,7 +66,8 @@ struct WaveReq {
     int W, H, spp, max_level, tile_compat;
     int retest;       // RTG_FLAG_FP64_RETEST: FP64 near-tie re-test of closest hits
     int bw;           // 8x8 pixel blocks per row
-    int n_samples;    // pixel blocks * 64 * spp
+    int row0;         // first row of pixel blocks this pass covers (a frame may take several passes)
+    int n_samples;    // pixel blocks of the pass * 64 * spp
     int sample_base;  // first level-0 queue slot of the request
     int scene;        // index into the batch's scene table
     float dpf;
@@ -109,7 +110,7 @@ __device__ inline bool sample_pixel(const WaveReq &R, int s, int &px, int &py, i
     const int lane = r / R.spp;
     si = r - lane * R.spp;
     px = (pb % R.bw) * 8 + (lane & 7);
-    py = (pb / R.bw) * 8 + (lane >> 3);
+    py = (R.row0 + pb / R.bw) * 8 + (lane >> 3);
     if (px >= R.W || py >= R.H) return false;
     if (R.tile_compat && (px >= ((R.W >> 6) << 6) || py >= ((R.H >> 6) << 6))) return false;
     return true;
@@ -436,11 +437,15 @@ __global__ __launch_bounds__(kBlockThreads) void k_combine(SceneView S0, WaveArg
 template <bool B>
 __global__ void k_resolve(WaveArgs A, const float4 *samples) {
     const WaveReq &R = req_at<B>(A, blockIdx.y);
-    const int npix = R.W * R.H;
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += gridDim.x * blockDim.x) {
+    // pixels of the pass: block rows row0 .. row0 + n_samples / (bw 64 spp)
+    const int y0 = R.row0 * 8;
+    const int y1 = R.bw > 0 ? min(R.H, y0 + 8 * (R.n_samples / (R.bw * 64 * R.spp))) : y0;
+    const int npix = R.W * (y1 - y0);
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < npix; q += gridDim.x * blockDim.x) {
+        const int p = y0 * R.W + q;
         const int px = p % R.W, py = p / R.W;
         if (R.tile_compat && (px >= ((R.W >> 6) << 6) || py >= ((R.H >> 6) << 6))) continue;
-        const int pb = (py >> 3) * R.bw + (px >> 3), lane = ((py & 7) << 3) | (px & 7);
+        const int pb = ((py >> 3) - R.row0) * R.bw + (px >> 3), lane = ((py & 7) << 3) | (px & 7);
         const int s0 = R.sample_base + (pb * 64 + lane) * R.spp;
         V3 acc = V3{0, 0, 0};
         for (int s = 0; s < R.spp; ++s) {
@@ -481,6 +486,8 @@ __global__ void k_chunk_req(const WaveReq *reqs, int n_reqs, int *chunk_req, int
 
 // Single frames with more samples run level by level (see launch_wavefront).
 constexpr int kNoSyncMaxSamples = 1 << 20;
+// Level-0 samples per pass of a frame whose rays branch (see launch_wavefront).
+constexpr int kPassSamples = 1 << 21;
 
 // Resident blocks per CU of a kernel at kBlockThreads threads (registers and
 // LDS permitting).
@@ -671,7 +678,6 @@ int launch_wavefront(const SceneRes &S, const rtg_params &p, int W, int H, const
     WaveReq &r = a.r0;
     r.W = W, r.H = H, r.spp = p.spp > 0 ? p.spp : 1, r.max_level = p.max_level, r.tile_compat = p.tile_compat;
     r.bw = (W + 7) / 8;
-    r.n_samples = r.bw * ((H + 7) / 8) * 64 * r.spp;
     r.sample_base = 0;
     r.scene = 0;
     r.dpf = (float)dp;
@@ -680,17 +686,28 @@ int launch_wavefront(const SceneRes &S, const rtg_params &p, int W, int H, const
     r.ax = ws.tables.as<float>();
     r.by = ws.tables.as<float>() + W;
     r.rgb = o.rgb, r.ids = o.ids, r.tprim = o.tprim;
-    a.n_samples = r.n_samples;
     a.counts = o.counts;
     a.retests = o.retests;
-    WavePlan pl = make_plan(r.n_samples, p.max_level, S.max_fanout, __builtin_popcount(S.lights_on), 1,
-                            A.no_sync_budget);
-    // A large frame keeps the GPU busy at every level; there, sizing each
-    // level from the previous level's count beats launching full grids
-    // against the worst-case bounds.
-    if (r.n_samples > kNoSyncMaxSamples) pl.no_sync = false;
-    if (pl.no_sync && (st = reserve_all(ws, pl, A.cus > 0 ? A.cus : 256)) != RTG_OK) return st;
-    return enqueue_wave<false>(S.view, a, pl, stream, ws, A, cancel_host, r.retest != 0);
+    // Frames whose ray trees can branch (refraction: two children per hit)
+    // run in passes of whole block rows, at most kPassSamples level-0
+    // samples each, so that the queues of all levels stay a few GB.
+    const int rows = (H + 7) / 8, per_row = r.bw * 64 * r.spp;
+    const int rows_per_pass = S.max_fanout >= 2 ? std::max(1, kPassSamples / per_row) : rows;
+    for (int row0 = 0; row0 < rows; row0 += rows_per_pass) {
+        r.row0 = row0;
+        r.n_samples = std::min(rows_per_pass, rows - row0) * per_row;
+        a.n_samples = r.n_samples;
+        WavePlan pl = make_plan(r.n_samples, p.max_level, S.max_fanout, __builtin_popcount(S.lights_on), 1,
+                                A.no_sync_budget);
+        // A large frame keeps the GPU busy at every level; there, sizing each
+        // level from the previous level's count beats launching full grids
+        // against the worst-case bounds.
+        if (r.n_samples > kNoSyncMaxSamples) pl.no_sync = false;
+        if (pl.no_sync && (st = reserve_all(ws, pl, A.cus > 0 ? A.cus : 256)) != RTG_OK) return st;
+        st = enqueue_wave<false>(S.view, a, pl, stream, ws, A, cancel_host, r.retest != 0);
+        if (st != RTG_OK || (cancel_host && *cancel_host)) return st;
+    }
+    return RTG_OK;
 }
 
 // Pinned staging for batch tables; replaced only while `stream` is idle.

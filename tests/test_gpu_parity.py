@@ -233,3 +233,25 @@ def test_fp64_error_statistics(rt_gpu, workload):
     a, b = fp64_stats(g, o64), fp64_stats(gf, o64)
     log(f"case: workload {workload} vs FP64 oracle: fp32 {a}; fp32+fp64 re-test {b}, re-tests {gf['stats'].retests_fp64}")
     assert a["ids_mismatch"] < 0.01 and b["ids_mismatch"] <= a["ids_mismatch"] + 1e-4
+
+
+def test_branching_frame_in_passes(rt_gpu):
+    """A refraction frame larger than one pass (2^21 level-0 samples) renders
+    in passes of block rows; pixels on both sides of the pass boundary match
+    the device code run on the CPU and the reference algorithm."""
+    from conftest import emul_render
+    rt = rt_gpu
+    W, H = 1024, 576  # 4 spp: 2,359,296 samples, two passes (block rows 0-63 | 64-71)
+    log(f"case: workload 5 (terrain /8, 64 sphere meshes) at {W}x{H} 4 spp, two passes")
+    hs = rt.HostScene.workload(5, width=W, height=H, detail=8, n_spheres=64, n_lights=16)
+    p = rt.params(W, H, max_level=8, spp=4, seed=5)
+    g = rt.DeviceScene(hs).render(p)
+    tx = (W + 63) // 64
+    tiles = np.array([7 * tx + 7, 8 * tx + 8, 8 * tx + 3], np.int32)  # rows 448-511 and 512-575 (pass boundary at 512)
+    e = emul_render(hs.desc_ptr, p, tiles=tiles)
+    m = tile_mask(W, H, tiles)
+    assert (g["rgb"][m] == e["rgb"][m]).all() and (g["ids"][m] == e["ids"][m]).all()
+    o = _oracle(hs, p, tiles=tiles)
+    s = compare_frames(g, o, region=m)
+    log(f"  {s}  rays {g['stats'].rays_total}  {g['stats'].render_ms:.1f} ms")
+    _check(s)
