@@ -219,7 +219,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--engine", default="auto", choices=["mega", "wave", "auto"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-tiles", type=int, default=32, help="1/N of the 64x64 tiles for the CPU baseline sample")
+    ap.add_argument("--cpu-tiles", type=int, default=2, help="1/N of the 64x64 tiles for the CPU baseline sample")
     ap.add_argument("--workload", type=int, default=2, choices=[2, 3, 4, 5],
                     help="2: the 1080p frame (default line); 3: 256 batched requests; 4: serving sweep; "
                          "5: 4K stress frame")
@@ -322,7 +322,10 @@ def main() -> int:
     threads = max(1, min(64, os.cpu_count() or 1))
     traffic_tiles = np.sort(rng.choice(tx * ty, size=(tx * ty) // 16, replace=False)).astype(np.int32)
     tm = traffic_model(osc.traffic(p, threads=threads, tiles=traffic_tiles))
-    tiles = traffic_tiles[:: max(1, args.cpu_tiles // 16)]
+    # CPU baseline sample: a seeded 1/N of the frame's tiles (about 10 s of
+    # oracle work on 16 host threads at N = 2)
+    tiles = np.sort(np.random.default_rng(12).choice(tx * ty, size=max(1, (tx * ty) // args.cpu_tiles),
+                                                     replace=False)).astype(np.int32)
     if dom == "k_occlude":
         unit, per_unit = "traced shadow ray", tm["per_shadow_ray"]
     elif dom == "k_extend":
