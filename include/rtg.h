@@ -165,7 +165,8 @@ typedef struct rtg_scene_desc {
 /* Per-frame render parameters: RayTracer::start(type, tnum) (RayTracer.cpp:614)
  * plus what the reference fixes implicitly. */
 typedef struct rtg_params {
-    int32_t mode;        /* RTG_MODE_* (only RTG_MODE_RAYTRACE / REFRACTTEST render the full tree) */
+    int32_t mode;        /* RTG_MODE_*: RAYTRACE / REFRACTTEST the full tree; REFLECTTEST without refraction;
+                            CHECK, DEPTHTEST .. SHADOWTEST the reference's primary-only debug modes */
     int32_t max_level;   /* RayTracer::maxLevel: levels 0..max_level are traced */
     int32_t width, height; /* 0: take the scene camera's */
     int32_t spp;         /* 1: the reference's single pixel-corner sample; 4: 2x2 jittered strata */

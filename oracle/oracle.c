@@ -101,6 +101,7 @@ struct oracle_scene_s {
 typedef struct {
     const oscene *S;
     int max_level;
+    int render_mode; /* rtg_params.mode: RTG_MODE_* */
     oracle_stats st;
     oracle_traffic *traffic; /* non-NULL: count canonical-BVH work per traced ray */
     int shadow_matters;      /* traffic only: the pending shadow ray can change the colour */
@@ -358,6 +359,7 @@ static void bvh_walk(const oscene *S, const double o[3], const double d[3], doub
 #define SFX(n) n##32
 #define R_SQRT sqrtf
 #define R_POW powf
+#define R_LOG logf
 #define R_EXP expf
 #define R_FABS fabsf
 #define R_MODF modff
@@ -497,6 +499,7 @@ static int occluded32(octx *cx, const RAY32 *ray, float dis, int64_t newobj) {
 #undef SFX
 #undef R_SQRT
 #undef R_POW
+#undef R_LOG
 #undef R_EXP
 #undef R_FABS
 #undef R_MODF
@@ -507,6 +510,7 @@ static int occluded32(octx *cx, const RAY32 *ray, float dis, int64_t newobj) {
 #define SFX(n) n##64
 #define R_SQRT sqrt
 #define R_POW pow
+#define R_LOG log
 #define R_EXP exp
 #define R_FABS fabs
 #define R_MODF modf
@@ -664,6 +668,7 @@ static int occluded64(octx *cx, const RAY64 *ray, double dis, int64_t newobj) {
 #undef SFX
 #undef R_SQRT
 #undef R_POW
+#undef R_LOG
 #undef R_EXP
 #undef R_FABS
 #undef R_MODF
@@ -995,6 +1000,16 @@ static void render_pixel(job_t *J, octx *cx, int x, int y) {
     const rtg_camera *cam = &J->S->d->camera;
     const int xcur = x - J->W / 2, ycur = y - J->H / 2;
     const int spp = J->p->spp > 0 ? J->p->spp : 1;
+    if (cx->render_mode == RTG_MODE_CHECK) { /* RTcheck (RayTracer.cpp:46-77): tile checkerboard, no rays */
+        const size_t pix = (size_t)y * J->W + x;
+        const uint8_t v = ((y / 64) & 1) == ((x / 64) & 1) ? 255 : 0;
+        if (J->rgb) J->rgb[3 * pix] = J->rgb[3 * pix + 1] = J->rgb[3 * pix + 2] = v;
+        if (J->rgbf) J->rgbf[3 * pix] = J->rgbf[3 * pix + 1] = J->rgbf[3 * pix + 2] = v ? 1.0f : 0.0f;
+        if (J->ids) J->ids[2 * pix] = J->ids[2 * pix + 1] = -1;
+        if (J->tprim) J->tprim[pix] = 1e20f;
+        if (J->flags) J->flags[pix] = 0;
+        return;
+    }
     float acc32[3] = {0, 0, 0};
     double acc64[3] = {0, 0, 0};
     for (int s = 0; s < spp; ++s) {
@@ -1025,7 +1040,9 @@ static void render_pixel(job_t *J, octx *cx, int x, int y) {
             ray.type = MY_RAY_BASERAY;
             ray.isInside = 0;
             HIT32 ph;
-            V32 c = rtfrac32(cx, J->zNear, J->zFar, &ray, 0, 1.0f, NO_OBJ, &ph);
+            const int dbg = cx->render_mode >= RTG_MODE_DEPTHTEST && cx->render_mode <= RTG_MODE_SHADOWTEST;
+            V32 c = dbg ? rtdebug32(cx, cx->render_mode, J->zNear, J->zFar, &ray, &ph)
+                        : rtfrac32(cx, J->zNear, J->zFar, &ray, 0, 1.0f, NO_OBJ, &ph);
             acc32[0] += c.x, acc32[1] += c.y, acc32[2] += c.z;
             if (s == 0) {
                 size_t pix = (size_t)y * J->W + x;
@@ -1046,7 +1063,9 @@ static void render_pixel(job_t *J, octx *cx, int x, int y) {
             ray.type = MY_RAY_BASERAY;
             ray.isInside = 0;
             HIT64 ph;
-            V64 c = rtfrac64(cx, cam->zNear, 1.4142135623730951 * cam->zFar, &ray, 0, 1.0, NO_OBJ, &ph);
+            const int dbg = cx->render_mode >= RTG_MODE_DEPTHTEST && cx->render_mode <= RTG_MODE_SHADOWTEST;
+            V64 c = dbg ? rtdebug64(cx, cx->render_mode, cam->zNear, 1.4142135623730951 * cam->zFar, &ray, &ph)
+                        : rtfrac64(cx, cam->zNear, 1.4142135623730951 * cam->zFar, &ray, 0, 1.0, NO_OBJ, &ph);
             acc64[0] += c.x, acc64[1] += c.y, acc64[2] += c.z;
             if (s == 0) {
                 size_t pix = (size_t)y * J->W + x;
@@ -1076,6 +1095,7 @@ static void *worker(void *arg) {
     memset(&cx, 0, sizeof(cx));
     cx.S = J->S;
     cx.max_level = J->p->max_level;
+    cx.render_mode = J->p->mode;
     oracle_traffic tr;
     memset(&tr, 0, sizeof(tr));
     if (J->traffic) cx.traffic = &tr;

@@ -99,6 +99,7 @@ inline std::string prepare_scene(const rtg_scene_desc &d, ScenePrep &P) {
         L.type = l.type, L.on = l.on;
         L.pos = prep_v3(l.position), L.la = prep_v3(l.ambient), L.ld = prep_v3(l.diffuse);
         L.ls = prep_v3(l.specular), L.att = prep_v3(l.attenuation);
+        L.pos_w = l.position.w;
     }
     P.texs.resize(d.n_textures > 0 ? d.n_textures : 1);
     for (int i = 0; i < d.n_textures; ++i) {

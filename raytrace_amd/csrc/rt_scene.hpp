@@ -60,6 +60,7 @@ struct DevMaterial {
 struct DevLight {
     int type, on;
     V3 pos, la, ld, ls, att;
+    float pos_w;  // Light::position.alpha (the material test's light-kind test)
 };
 
 struct DevTexture {

@@ -259,9 +259,11 @@ RTG_HD int id_prim(const SceneView &S, const Hit &h) {
 // One sample of the megakernel: the ray tree of RTfrac evaluated depth first
 // with an explicit frame stack; children are composed into their parent in
 // the reference's order.
+// `no_refract`: the reflect test (RTflec, RayTracer.cpp:332-448), RTfrac
+// without its refraction branch.
 template <bool kRetest = false, class Stack>
 RTG_HD V3 render_sample(const SceneView &S, int max_level, const TraceRay &primary, Stack &st, RayCounts &rc,
-                        PrimaryOut &po) {
+                        PrimaryOut &po, bool no_refract = false) {
     struct Frame {
         V3 c, P, N, D;
         float t, bwc, mtlrfr, rfr, rl, rr;
@@ -321,7 +323,7 @@ RTG_HD V3 render_sample(const SceneView &S, int max_level, const TraceRay &prima
                 f.stage = 0;
                 f.id = hr.id;
                 f.rl = S.mtls[hr.mtl].reflect;
-                f.rr = S.mtls[hr.mtl].refract;
+                f.rr = no_refract ? 0.0f : S.mtls[hr.mtl].refract;
                 produced = true;
             }
         }
@@ -387,6 +389,78 @@ RTG_HD V3 render_sample(const SceneView &S, int max_level, const TraceRay &prima
         }
         if (!descend) return ret;
     }
+}
+
+// Debug render modes (RayTracer.cpp:46-330): one primary closest hit,
+// coloured by the mode.  RTdepth: Color::set (3DElement.cpp:451-462), log
+// depth between zNear and zFar; RTnorm: 0.5 (N + 1); RTtex: the texel, or
+// 0.588 grey without texture; RTmtl: lights without shadows, the light kind
+// by position.w > 1e-6 and attenuation x + y sqrt(d) + z d; RTshd: lights
+// with shadow rays.  RTmtl and RTshd add the environment ambient after the
+// lights (RTfrac adds it first).
+template <class Stack>
+RTG_HD V3 debug_sample(const SceneView &S, int mode, const TraceRay &q, Stack &st, RayCounts &rc, PrimaryOut &po) {
+    rc.primary++;
+    const Hit h = trace_closest(S, q, st);
+    po = PrimaryOut{h.obj, id_prim(S, h), h.t};
+    const float zNear = S.cam.zNear, zFar = S.cam.zFar;
+    if (mode == 0x2) {  // MY_MODEL_DEPTHTEST
+        const float d = h.t;
+        if (d <= zNear) return V3{1.0f, 0.0f, 0.0f};
+        if (d >= zFar) return V3{0.0f, 0.0f, 0.0f};
+        const float after = logf(d), mx = logf(zFar);
+        const float g = (mx - after) / mx;
+        return V3{g, g, g};
+    }
+    if (h.kind == HIT_NONE || h.t > zFar) return V3{0.0f, 0.0f, 0.0f};
+    if (h.t < zNear) return V3{1.0f, 1.0f, 1.0f};
+    const HitRec hr = make_hitrec(S, q, h);
+    if (mode == 0x3)  // MY_MODEL_NORMALTEST
+        return V3{0.5f * (hr.N.x + 1.0f), 0.5f * (hr.N.y + 1.0f), 0.5f * (hr.N.z + 1.0f)};
+    if (mode == 0x4)  // MY_MODEL_TEXTURETEST
+        return hr.tex >= 0 && hr.tex < S.n_texs ? texel(S, hr.tex, hr.tu, hr.tv) : V3{0.588f, 0.588f, 0.588f};
+    const DevMaterial &m = S.mtls[hr.mtl];
+    const V3 vc = texel(S, hr.tex, hr.tu, hr.tv);
+    V3 vd = V3{0, 0, 0}, va = V3{0, 0, 0}, vs = V3{0, 0, 0};
+    for (int li = 0; li < S.n_lights; ++li) {
+        const DevLight &L = S.lights[li];
+        if (!L.on) continue;
+        LightTerm lt;
+        if (mode == 0x5) {  // MY_MODEL_MATERIALTEST
+            if ((double)L.pos_w > 1e-6) {
+                const V3 p2l_v = L.pos - hr.P;
+                const float d2 = dot(p2l_v, p2l_v);
+                const float step = (L.att.x + L.att.y * sqrtf(d2)) + L.att.z * d2;
+                const float lum = 1.0f / step;
+                lt.la = L.la * lum, lt.ld = L.ld * lum, lt.ls = L.ls * lum;
+                lt.p2l = normalize(p2l_v);
+            } else {
+                lt.la = L.la, lt.ld = L.ld, lt.ls = L.ls;
+                lt.p2l = normalize(L.pos);
+            }
+            lt.dis = 1e10f;
+        } else {
+            lt = light_term(L, hr.P);
+        }
+        va = va + mixmul(m.ka, lt.la);
+        const LightDots ld = light_dots(hr.N, q.r.d, lt);
+        if (mode == 0x6) {  // MY_MODEL_SHADOWTEST
+            rc.shadow++;
+            if (!ld.matters()) continue;
+            TraceRay sr;
+            sr.r.o = hr.P;
+            sr.r.d = lt.p2l;
+            sr.type = RAY_SHADOW;
+            sr.inside = 0;
+            sr.src = hr.id;
+            sr.self_test = self_testable(S, sr.r, hr.id, lt.dis);
+            if (trace_any(S, sr, lt.dis, st)) continue;
+        }
+        if (ld.n_l > 0) vd = vd + mixmul(m.kd, lt.ld) * ld.n_l;
+        if (ld.n_h > 0) vs = vs + mixmul(m.ks, lt.ls) * powf(ld.n_h, m.shiness);
+    }
+    va = va + mixmul(m.ka, S.env);
+    return mixmul(vc, vd + va) + vs;
 }
 
 }  // namespace rtg

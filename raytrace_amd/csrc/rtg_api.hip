@@ -188,12 +188,18 @@ static int render_device(SceneRes &S, const rtg_params &p, int W, int H, const R
     if (o.ids) k_fill_u8<<<1024, 256, 0, stream>>>(reinterpret_cast<uint8_t *>(o.ids), npix * 8, 0xff);
     k_zero_u64<<<1, 64, 0, stream>>>(o.counts, 4);
     if (o.retests) k_zero_u64<<<1, 64, 0, stream>>>(o.retests, 1);
-    if (p.mode != RTG_MODE_RAYTRACE && p.mode != RTG_MODE_REFRACTTEST && p.mode != 0)
-        return fail(RTG_E_INVALID, "render mode not supported by this build");
+    if (!(p.mode == 0 || (p.mode >= RTG_MODE_CHECK && p.mode <= RTG_MODE_REFRACTTEST) || p.mode == RTG_MODE_RAYTRACE))
+        return fail(RTG_E_INVALID, "unknown render mode");
     // AUTO: the wavefront engine.  The megakernel is slower on every large or
     // deep frame (its per-thread ray trees diverge); it only wins on small,
-    // shallow frames that take well under a millisecond either way.
+    // shallow frames that take well under a millisecond either way.  The
+    // primary-only debug modes (checker, depth, normal, texture, material,
+    // shadow) run in the megakernel: one closest hit per sample, no tree.
     int engine = p.engine == RTG_ENGINE_AUTO ? RTG_ENGINE_WAVEFRONT : p.engine;
+    if (p.mode >= RTG_MODE_CHECK && p.mode <= RTG_MODE_SHADOWTEST) {
+        if (p.flags & RTG_FLAG_FP64_RETEST) return fail(RTG_E_INVALID, "the FP64 re-test applies to ray-traced modes");
+        engine = RTG_ENGINE_MEGAKERNEL;
+    }
     if (engine == RTG_ENGINE_MEGAKERNEL && (p.flags & RTG_FLAG_FP64_RETEST))
         return fail(RTG_E_INVALID, "the FP64 near-tie re-test needs the wavefront engine");
     int st = A.ws.order_before(stream);
@@ -514,8 +520,9 @@ int rtg_render_batch(const rtg_request *reqs, int n, rtg_stats *stats) {
         int W, H;
         int st = resolve_size(q.scene->res, &q.params, W, H);
         if (st != RTG_OK) return st;
-        if (q.params.mode != RTG_MODE_RAYTRACE && q.params.mode != RTG_MODE_REFRACTTEST && q.params.mode != 0)
-            return fail(RTG_E_INVALID, "render mode not supported by this build");
+        if (q.params.mode != RTG_MODE_RAYTRACE && q.params.mode != RTG_MODE_REFRACTTEST &&
+            q.params.mode != RTG_MODE_REFLECTTEST && q.params.mode != 0)
+            return fail(RTG_E_INVALID, "batches render the ray-traced modes (debug modes: one frame per call)");
         eager = eager || (q.params.flags & RTG_FLAG_EAGER) != 0;
         items[i] = BatchItem{&q.scene->res, q.params, W, H, nullptr, nullptr, nullptr};
         off_rgb[i] = rgb_bytes;

@@ -11,6 +11,7 @@ namespace rtg {
 
 struct MegaArgs {
     int W, H, spp, max_level, tile_compat;
+    int mode;  // RTG_MODE_*: the full ray tree, the reflect test, or a debug mode
     int bw, nblocks;
     float dpf;
     uint64_t seed;
@@ -61,14 +62,21 @@ __global__ __launch_bounds__(kBlockThreads) void k_megakernel(SceneView S, MegaA
         if (A.tile_compat && (px >= tw || py >= th)) continue;
         V3 acc = V3{0, 0, 0};
         PrimaryOut po{-1, -1, kMiss};
-        for (int s = 0; s < A.spp; ++s) {
-            const TraceRay q = primary_ray(S, A.ax, A.by, A.W, px, py, s, A.spp, A.dpf, A.seed);
-            PrimaryOut p0;
-            const V3 c = render_sample(S, A.max_level, q, st, rc, p0);
-            if (s == 0) po = p0;
-            acc = acc + c;
+        if (A.mode == RTG_MODE_CHECK) {  // RTcheck (RayTracer.cpp:46-77): tile checkerboard, no rays
+            const float v = ((py >> 6) & 1) == ((px >> 6) & 1) ? 1.0f : 0.0f;
+            acc = V3{v, v, v};
+        } else {
+            const bool debug = A.mode >= RTG_MODE_DEPTHTEST && A.mode <= RTG_MODE_SHADOWTEST;
+            for (int s = 0; s < A.spp; ++s) {
+                const TraceRay q = primary_ray(S, A.ax, A.by, A.W, px, py, s, A.spp, A.dpf, A.seed);
+                PrimaryOut p0;
+                const V3 c = debug ? debug_sample(S, A.mode, q, st, rc, p0)
+                                   : render_sample(S, A.max_level, q, st, rc, p0, A.mode == RTG_MODE_REFLECTTEST);
+                if (s == 0) po = p0;
+                acc = acc + c;
+            }
         }
-        if (A.spp > 1) acc = acc * (1.0f / (float)A.spp);
+        if (A.spp > 1 && A.mode != RTG_MODE_CHECK) acc = acc * (1.0f / (float)A.spp);
         const size_t pix = (size_t)py * A.W + px;
         A.rgb[3 * pix + 0] = quantize(acc.x);
         A.rgb[3 * pix + 1] = quantize(acc.y);
@@ -124,6 +132,7 @@ int launch_megakernel(const SceneRes &S, const rtg_params &p, int W, int H, cons
     if (A.ws.grow(A.ws.spill, spill_ints(blocks) * sizeof(int))) return fail(RTG_E_NOMEM, "spill");
     MegaArgs a;
     a.W = W, a.H = H, a.spp = p.spp > 0 ? p.spp : 1, a.max_level = p.max_level, a.tile_compat = p.tile_compat;
+    a.mode = p.mode;
     a.bw = (W + 15) / 16;
     a.nblocks = a.bw * ((H + 7) / 8);
     a.dpf = (float)dp;

@@ -135,8 +135,9 @@ int serve_run(rtg_serve_request *reqs, int n, const rtg_serve_opts &o, rtg_serve
         if (q.scene == nullptr || q.rgb == nullptr) return fail(RTG_E_INVALID, "request without scene or rgb");
         int st = resolve_size(q.scene->res, &q.params, info[i].W, info[i].H);
         if (st != RTG_OK) return st;
-        if (q.params.mode != RTG_MODE_RAYTRACE && q.params.mode != RTG_MODE_REFRACTTEST && q.params.mode != 0)
-            return fail(RTG_E_INVALID, "render mode not supported by this build");
+        if (q.params.mode != RTG_MODE_RAYTRACE && q.params.mode != RTG_MODE_REFRACTTEST &&
+            q.params.mode != RTG_MODE_REFLECTTEST && q.params.mode != 0)
+            return fail(RTG_E_INVALID, "the serving loop renders the ray-traced modes");
         info[i].samples = frame_samples(info[i].W, info[i].H, q.params.spp);
         info[i].fanout = q.scene->res.max_fanout;
         info[i].n_on = __builtin_popcount(q.scene->res.lights_on);

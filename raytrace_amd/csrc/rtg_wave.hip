@@ -65,6 +65,7 @@ constexpr int kCtrWords = (int)(sizeof(LevelCtr) / sizeof(unsigned));
 struct WaveReq {
     int W, H, spp, max_level, tile_compat;
     int retest;       // RTG_FLAG_FP64_RETEST: FP64 near-tie re-test of closest hits
+    int no_refract;   // the reflect test (RTflec, RayTracer.cpp:332-448): no refraction branch
     int bw;           // 8x8 pixel blocks per row
     int row0;         // first row of pixel blocks this pass covers (a frame may take several passes)
     int n_samples;    // pixel blocks of the pass * 64 * spp
@@ -302,17 +303,18 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_shade(SceneView S0, WaveAr
                 const HitRec hr = make_hitrec(S, q, h);
                 const V3 vc = texel(S, hr.tex, hr.tu, hr.tv);
                 const DevMaterial &m = S.mtls[hr.mtl];
+                const float m_refract = R.no_refract ? 0.0f : m.refract;
                 int tir = 0;
                 if (m.reflect > 0.01f && k + 1 <= R.max_level && bwc * m.reflect >= 1e-5f) {
                     c0 = make_wray(reflect_ray(q, hr), bwc * m.reflect, 1.0f, 2 * i, r);
                     n_child = 1;
                 }
-                if (m.refract > 0.01f) {
+                if (m_refract > 0.01f) {
                     TraceRay cq;
                     if (!refract_ray(q, mtlrfr, hr, cq)) {
                         tir = 1;
-                    } else if (k + 1 <= R.max_level && bwc * m.refract >= 1e-5f) {
-                        const WRay cw = make_wray(cq, bwc * m.refract, hr.rfr, 2 * i + 1, r);
+                    } else if (k + 1 <= R.max_level && bwc * m_refract >= 1e-5f) {
+                        const WRay cw = make_wray(cq, bwc * m_refract, hr.rfr, 2 * i + 1, r);
                         if (n_child == 0)
                             c0 = cw;
                         else
@@ -413,7 +415,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_combine(SceneView S0, WaveArg
         const WNode nd = nodes[i];
         float4 out = make_float4(0.0f, 0.0f, 0.0f, kMiss);
         if ((nd.bits >> 27) & 1) {
-            const SceneView &S = scene_at<B>(S0, A, req_at<B>(A, node_req(nd)).scene);
+            const WaveReq &R = req_at<B>(A, node_req(nd));
+            const SceneView &S = scene_at<B>(S0, A, R.scene);
             const int mtl = node_mtl(nd);
             const unsigned mask = masks[i];
             const V3 L = shade_local(S, V3{nd.dx, nd.dy, nd.dz}, V3{nd.px, nd.py, nd.pz}, V3{nd.nx, nd.ny, nd.nz},
@@ -421,8 +424,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_combine(SceneView S0, WaveArg
                                      [&](int li, const LightTerm &, bool) -> bool { return (mask >> li) & 1u; });
             const DevMaterial &m = S.mtls[mtl];
             const float4 a = slots[2 * i], b = slots[2 * i + 1];
-            const V3 col = compose(L, m.reflect, m.refract, (nd.bits >> 29) & 1, (nd.bits >> 28) & 1, m.kd,
-                                   V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, b.w);
+            const V3 col = compose(L, m.reflect, R.no_refract ? 0.0f : m.refract, (nd.bits >> 29) & 1,
+                                   (nd.bits >> 28) & 1, m.kd, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, b.w);
             out = make_float4(col.x, col.y, col.z, nd.t);
         }
         if (k == 0)
@@ -683,6 +686,7 @@ int launch_wavefront(const SceneRes &S, const rtg_params &p, int W, int H, const
     r.dpf = (float)dp;
     r.seed = p.seed;
     r.retest = (p.flags & RTG_FLAG_FP64_RETEST) != 0;
+    r.no_refract = p.mode == RTG_MODE_REFLECTTEST;
     r.ax = ws.tables.as<float>();
     r.by = ws.tables.as<float>() + W;
     r.rgb = o.rgb, r.ids = o.ids, r.tprim = o.tprim;
@@ -761,6 +765,7 @@ int launch_batch(const BatchItem *items, int n, unsigned long long *counts, unsi
         r.scene = scene_of[i];
         r.seed = it.params.seed;
         r.retest = (it.params.flags & RTG_FLAG_FP64_RETEST) != 0;
+        r.no_refract = it.params.mode == RTG_MODE_REFLECTTEST;
         any_retest = any_retest || r.retest;
         r.rgb = it.rgb, r.ids = it.ids, r.tprim = it.tprim;
         total += (size_t)r.n_samples;

@@ -211,12 +211,22 @@ int emul_render(const rtg_scene_desc *d, const rtg_params *p, int threads, const
                         q.src = no_src();
                         q.self_test = 0;
                         PrimaryOut p0;
-                        const V3 c = retest ? render_sample<true>(H.view, p->max_level, q, st, rc, p0)
-                                            : render_sample<false>(H.view, p->max_level, q, st, rc, p0);
+                        const bool no_refract = p->mode == RTG_MODE_REFLECTTEST;
+                        V3 c;
+                        if (p->mode == RTG_MODE_CHECK) {
+                            const float v = ((y >> 6) & 1) == ((x >> 6) & 1) ? 1.0f : 0.0f;
+                            c = V3{v, v, v};
+                            p0 = PrimaryOut{-1, -1, kMiss};
+                        } else if (p->mode >= RTG_MODE_DEPTHTEST && p->mode <= RTG_MODE_SHADOWTEST) {
+                            c = debug_sample(H.view, p->mode, q, st, rc, p0);
+                        } else {
+                            c = retest ? render_sample<true>(H.view, p->max_level, q, st, rc, p0, no_refract)
+                                       : render_sample<false>(H.view, p->max_level, q, st, rc, p0, no_refract);
+                        }
                         if (s == 0) po = p0;
                         acc = acc + c;
                     }
-                    if (spp > 1) acc = acc * (1.0f / (float)spp);
+                    if (spp > 1 && p->mode != RTG_MODE_CHECK) acc = acc * (1.0f / (float)spp);
                     const size_t pix = (size_t)y * W + x;
                     rgb[3 * pix] = quantize(acc.x);
                     rgb[3 * pix + 1] = quantize(acc.y);

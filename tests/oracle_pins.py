@@ -305,15 +305,15 @@ def _known_scene(extra=()):
     return hs
 
 
-def _centre_colour(lib, hs, max_level):
+def _centre_colour(lib, hs, max_level, mode=0x80):
     import raytrace_amd as rt
     from oracle import pyoracle as po
     W, H = 1120, 630  # Camera default size
     tx = (W + 63) // 64
     cx, cy = W // 2, H // 2
     tile = (cy // 64) * tx + cx // 64
-    out = po.OracleScene(hs.desc_ptr, lib=lib).render(rt.params(W, H, max_level=max_level), po.ORACLE_REF32,
-                                                      threads=2, tiles=[tile])
+    out = po.OracleScene(hs.desc_ptr, lib=lib).render(rt.params(W, H, max_level=max_level, mode=mode),
+                                                      po.ORACLE_REF32, threads=2, tiles=[tile])
     return np.asarray(out["rgbf"][cy, cx], np.float64), out
 
 
@@ -428,6 +428,42 @@ def pin_ray_counts(lib):
     assert st.rays_reflect == 0 and st.rays_refract == 0
 
 
+def pin_debug_modes(lib):
+    """Known answers of the reference's debug modes at the centre pixel (hit at
+    t = 14 on the unit sphere, N = (0,0,1)): depth (ln zFar' - ln 14) / ln zFar'
+    with zFar' = (float)(sqrt(2) 100) (Color::set); normal 0.5 (N + 1); texture
+    0.588 grey without a texture; material = unshadowed local shading even with
+    a blocker; shadow = ambient only with the blocker; reflect test ignores
+    refraction; checker = the centre tile's parity."""
+    from raytrace_amd import _capi
+    zfar = float(np.float32(np.sqrt(2.0) * 100.0))
+    hs = _known_scene()
+    got, _ = _centre_colour(lib, hs, 0, mode=0x2)
+    g = (np.log(zfar) - np.log(14.0)) / np.log(zfar)
+    assert np.allclose(got, [g, g, g], rtol=2e-6), (got, g)
+    got, _ = _centre_colour(lib, hs, 0, mode=0x3)
+    assert np.allclose(got, [0.5, 0.5, 1.0], atol=1e-6), got
+    got, _ = _centre_colour(lib, hs, 0, mode=0x4)
+    assert np.allclose(got, [0.588, 0.588, 0.588], atol=1e-6), got
+    blocker = [((1.5, 4, 3), 0.5, _mat(KA, KD, KS, SHIN))]
+    P, N, D = np.array([0, 4.0, 1]), np.array([0, 0, 1.0]), np.array([0, 0, -1.0])
+    hb = _known_scene(blocker)
+    _capi.check(_capi.load().rtg_host_set_material(hb.h, 0, C.byref(_mat(KA, KD, KS, SHIN))), "ChgMtl")
+    got, _ = _centre_colour(lib, hb, 0, mode=0x5)
+    assert np.allclose(got, _local(P, N, D, KA, KD, KS, SHIN), rtol=1e-5, atol=1e-6), got
+    got, _ = _centre_colour(lib, hb, 0, mode=0x6)
+    assert np.allclose(got, _local(P, N, D, KA, KD, KS, SHIN, True), rtol=1e-5, atol=1e-6), got
+    hr = _known_scene()
+    _capi.check(_capi.load().rtg_host_set_material(hr.h, 0, C.byref(_mat(KA, KD, KS, SHIN, refract=0.5, rfr=1.5))),
+                "ChgMtl")
+    got, _ = _centre_colour(lib, hr, 2, mode=0x7)
+    assert np.allclose(got, _local(P, N, D, KA, KD, KS, SHIN), rtol=1e-5, atol=1e-6), got
+    got, out = _centre_colour(lib, hs, 0, mode=0x1)
+    cx, cy = 1120 // 2, 630 // 2
+    v = 1.0 if ((cx // 64) & 1) == ((cy // 64) & 1) else 0.0
+    assert np.allclose(got, [v, v, v]) and out["stats"].rays() == 0, got
+
+
 ALL_PINS = [pin_dot_is_dpps, pin_normalize_is_sse, pin_raygen, pin_sphere, pin_plane, pin_triangle,
             pin_border_tests, pin_quantize, pin_local_shading, pin_shadow, pin_reflection, pin_refraction_beer,
-            pin_tie_rule, pin_ray_counts]
+            pin_tie_rule, pin_ray_counts, pin_debug_modes]

@@ -132,3 +132,23 @@ def test_fp64_retest_changes_only_band_pixels(workload):
     changed = ~(ef["ids"] == e["ids"]).all(-1)
     assert ((o32["flags"][changed] & 3) != 0).all()  # every change is a band pixel
     assert fp64_stats(ef, o64)["ids_mismatch"] <= fp64_stats(e, o64)["ids_mismatch"]
+
+
+@pytest.mark.parametrize("mode", [0x1, 0x2, 0x3, 0x4, 0x5, 0x6, 0x7])
+def test_debug_modes(mode):
+    """The reference's debug render modes (RayTracer.cpp:46-448): checker,
+    depth, normal, texture, material, shadow, reflect test.  Workload 1 has a
+    textured plane and reflecting spheres; the reflect test also runs on a
+    refraction scene, where it must drop the refraction branch."""
+    hs = rt.HostScene.workload(1)
+    p = rt.params(640, 480, max_level=3, mode=mode)
+    o, e = _run(hs, p)
+    _assert_parity(o, e)
+    assert list(e["counts"]) == [o["stats"].rays_primary, o["stats"].rays_shadow, o["stats"].rays_reflect,
+                                 o["stats"].rays_refract]
+    if mode == 0x7:
+        hs = rt.HostScene.workload(5, width=256, height=144, detail=32, n_spheres=24, n_lights=16)
+        p = rt.params(256, 144, max_level=6, mode=mode)
+        o, e = _run(hs, p)
+        _assert_parity(o, e, exact=False)
+        assert o["stats"].rays_refract == 0 and e["counts"][3] == 0

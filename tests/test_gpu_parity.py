@@ -255,3 +255,33 @@ def test_branching_frame_in_passes(rt_gpu):
     s = compare_frames(g, o, region=m)
     log(f"  {s}  rays {g['stats'].rays_total}  {g['stats'].render_ms:.1f} ms")
     _check(s)
+
+
+@pytest.mark.parametrize("mode", [0x1, 0x2, 0x3, 0x4, 0x5, 0x6, 0x7])
+def test_debug_modes_on_device(rt_gpu, mode):
+    """RayTracer::start(type) debug modes on the GPU against the reference
+    algorithm (and bit for bit against the device code run on the CPU)."""
+    from conftest import emul_render
+    rt = rt_gpu
+    log(f"case: render mode {mode:#x} on workload 1 (and on a refraction scene for the reflect test)")
+    cases = [(rt.HostScene.workload(1), dict(width=640, height=480, max_level=3))]
+    if mode == 0x7:
+        cases.append((rt.HostScene.workload(5, width=256, height=144, detail=32, n_spheres=24, n_lights=16),
+                      dict(width=256, height=144, max_level=6)))
+    for hs, pr in cases:
+        ds = rt.DeviceScene(hs)
+        for engine in ([1, 2] if mode == 0x7 else [0]):
+            p = rt.params(mode=mode, engine=engine, **pr)
+            g = ds.render(p)
+            o = _oracle(hs, p)
+            s = compare_frames(g, o)
+            _check(s)
+            e = emul_render(hs.desc_ptr, p)
+            assert (g["rgb"] == e["rgb"]).all() and (g["ids"] == e["ids"]).all()
+            st, ost = g["stats"], o["stats"]
+            assert (st.rays_primary, st.rays_shadow, st.rays_reflect, st.rays_refract) == (
+                ost.rays_primary, ost.rays_shadow, ost.rays_reflect, ost.rays_refract)
+    # through the drop-in class: whole 64x64 tiles only, the rest stays 127
+    buf, _ = cases[0][0].raytrace(mode, 3)
+    img = buf[:640 * 480 * 3].reshape(480, 640, 3)
+    assert (img[448:] == 127).all()
