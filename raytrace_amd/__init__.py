@@ -112,6 +112,16 @@ class HostScene:
     def switch(self, type_: int, num: int, show: bool) -> int:
         return check(self._lib.rtg_host_switch(self.h, type_, num, int(show)), "Switch")
 
+    def set_material(self, num: int, ka, kd, ks, shiness: float, reflect: float = 0.0, refract: float = 0.0,
+                     rfr: float = 1.0) -> None:
+        """Scene::ChgMtl(num, Material) (Scene.cpp:248-280) with the given components."""
+        m = _capi.rtg_material()
+        m.ambient, m.diffuse, m.specular = (_capi.rtg_vec4(*ka, 1.0), _capi.rtg_vec4(*kd, 1.0),
+                                            _capi.rtg_vec4(*ks, 1.0))
+        m.emission = _capi.rtg_vec4(0, 0, 0, 1)
+        m.shiness, m.reflect, m.refract, m.rfr, m.tex = shiness, reflect, refract, rfr, -1
+        check(self._lib.rtg_host_set_material(self.h, num, C.byref(m)), "ChgMtl")
+
     def set_position(self, num: int, pos) -> None:
         check(self._lib.rtg_host_set_position(self.h, num, _capi.floats(*pos)), "position")
 

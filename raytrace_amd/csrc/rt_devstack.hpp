@@ -29,6 +29,8 @@ struct DevStack {
         return sp < kLdsStack ? lds[sp * kBlockThreads] : spill[(sp - kLdsStack) * gstride];
     }
     __device__ bool empty() const { return sp == 0; }
+    __device__ void on_node() {}
+    __device__ void on_prim() {}
 };
 
 __device__ inline DevStack make_stack(int *lds_base, int *spill) {

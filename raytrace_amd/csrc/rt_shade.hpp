@@ -297,7 +297,9 @@ RTG_HD V3 render_sample(const SceneView &S, int max_level, const TraceRay &prima
             if (h.kind == HIT_NONE || h.t > zFar || h.t < zNear) {
                 ret = V3{0, 0, 0}, ret_a = kMiss;
             } else {
-                const HitRec hr = make_hitrec(S, q, h);
+                HitRec hr = make_hitrec(S, q, h);
+                const SrcId cs = child_src(S, q, h, st);
+                if (cs.obj != -2) hr.id = cs;
                 const V3 L = local_color(S, q.r.d, hr, [&](int, const LightTerm &lt, bool matters) -> bool {
                     rc.shadow++;
                     if (!matters) return false;

@@ -67,6 +67,8 @@ struct ArrayStack {
     RTG_HD void push(int v) { d[sp++] = v; }
     RTG_HD int pop() { return d[--sp]; }
     RTG_HD bool empty() const { return sp == 0; }
+    RTG_HD void on_node() {}
+    RTG_HD void on_prim() {}
 };
 
 // Visit the primitives of every leaf the segment [0, tmax] reaches, nearer
@@ -82,6 +84,7 @@ RTG_HD void bvh_walk(const SceneView &S, const Ray &r, float &tmax, Stack &st, V
     int node = S.root;
     for (;;) {
         while (!is_leaf(node)) {
+            st.on_node();
             const BvhNode &n = S.nodes[node];
             const float4 a = n.xy0, b = n.xy1, z = n.z01;
             const int4 l = n.link;
@@ -102,8 +105,10 @@ RTG_HD void bvh_walk(const SceneView &S, const Ray &r, float &tmax, Stack &st, V
             }
         }
         const int first = leaf_first(node), count = leaf_count(node);
-        for (int i = 0; i < count; ++i)
+        for (int i = 0; i < count; ++i) {
+            st.on_prim();
             if (visit(first + i, tmax)) return;
+        }
         if (st.empty()) return;
         node = st.pop();
     }
@@ -153,9 +158,11 @@ RTG_HD bool ballplane_center(const DevObject &o, int k, V3 &c) {
 
 // Objects outside the BVH, in object order.  Returns true for an any-hit
 // query when blocked.
-RTG_HD bool brute_objects(const SceneView &S, const TraceRay &q, bool anyhit, float dis, Hit &best) {
+RTG_HD bool brute_objects(const SceneView &S, const TraceRay &q, bool anyhit, float dis, Hit &best,
+                          int before = 0x7fffffff) {
     for (int k = 0; k < S.n_brute; ++k) {
         const int oi = S.brute[k];
+        if (oi >= before) break;  // `brute` is in object order
         const DevObject &o = S.objs[oi];
         const float lim = anyhit ? dis : best.t;
         if (o.type == OBJ_PLANE) {
@@ -290,11 +297,13 @@ RTG_HD bool retest64(const SceneView &S, const TraceRay &q, int pi, double &t64,
 }
 
 // Closest hit (RayTracer.cpp:455-465).  kRetest: FP64 near-tie re-test (see
-// above); *retested counts the queries that were re-tested.
-template <bool kRetest = false, class Stack>
-RTG_HD Hit trace_closest(const SceneView &S, const TraceRay &q, Stack &st, int *retested = nullptr) {
+// above); *retested counts the queries that were re-tested.  kBefore: only
+// objects with index < `before` take part (see child_src).
+template <bool kRetest = false, bool kBefore = false, class Stack>
+RTG_HD Hit trace_closest(const SceneView &S, const TraceRay &q, Stack &st, int *retested = nullptr,
+                         int before = 0x7fffffff) {
     Hit best{kMiss, HIT_NONE, -1, -1, 0, 0, 0};
-    brute_objects(S, q, false, 0.0f, best);
+    brute_objects(S, q, false, 0.0f, best, kBefore ? before : 0x7fffffff);
     uint64_t bkey = best.kind != HIT_NONE ? hit_key(best.obj, best.prim) : ~0ull;
     float tmax = best.t;
     int bpi = -1, pi2 = -1;  // primitive slots of the FP32 winner and runner-up (BVH primitives)
@@ -318,6 +327,7 @@ RTG_HD Hit trace_closest(const SceneView &S, const TraceRay &q, Stack &st, int *
         if (tag >= 0) {  // triangle, tag = global index
             const float4 B = S.prims[3 * pi + 1], C = S.prims[3 * pi + 2];
             const int obj = as_int(B.w);
+            if (kBefore && obj >= before) return false;
             if (tag == q.src.prim && obj == q.src.obj && !q.self_test) return false;
             const V3 e1 = V3{B.x, B.y, B.z}, e2 = V3{C.x, C.y, C.z};
             float w, u, v, t;
@@ -345,6 +355,7 @@ RTG_HD Hit trace_closest(const SceneView &S, const TraceRay &q, Stack &st, int *
             if (t < kMiss) take(t, hit_key(obj, tag), pi, Hit{t, HIT_TRI, obj, tag, w, u, v}, tm);
         } else {  // sphere, tag = ~object
             const int obj = ~tag;
+            if (kBefore && obj >= before) return false;
             const float4 B = S.prims[3 * pi + 1];
             float t;
             if (q.src.obj == obj) {
@@ -389,6 +400,31 @@ RTG_HD Hit trace_closest(const SceneView &S, const TraceRay &q, Stack &st, int *
         }
     }
     return best;
+}
+
+// Identity RTfrac hands to the children and shadow rays of a hit: its
+// `newobj` (RayTracer.cpp:455-465), the last object in scan order whose hit
+// changed HitRes::obj away from the ray's source identity.  That is the
+// winner, except when the winner is the source object itself hit again from
+// inside (a sphere or BallPlane-sphere exit): then it is the object that was
+// the running closest hit before the source object's turn in the scan, i.e.
+// the closest hit among the objects scanned before it, or the source identity
+// when none of them was hit.  Returns obj == -2 when it is the winner.
+RTG_HD inline bool is_self_rehit(const TraceRay &q, const Hit &h) {
+    return q.src.obj >= 0 && h.obj == q.src.obj &&
+           (h.kind == HIT_SPHERE || (h.kind == HIT_BALL && h.prim == q.src.prim));
+}
+// The identity from `b`, the closest hit among the objects before the source.
+RTG_HD inline SrcId child_src_of(const SceneView &S, const TraceRay &q, const Hit &b) {
+    if (b.kind == HIT_NONE) return q.src;
+    SrcId id{b.obj, b.kind == HIT_TRI || b.kind == HIT_BALL ? b.prim : -1, -1};
+    if (b.kind == HIT_TRI) id.bin = record_bin(S, q.r, q.src, b.prim);
+    return id;
+}
+template <class Stack>
+RTG_HD SrcId child_src(const SceneView &S, const TraceRay &q, const Hit &h, Stack &st) {
+    if (!is_self_rehit(q, h)) return SrcId{-2, -1, -1};
+    return child_src_of(S, q, trace_closest<false, true>(S, q, st, nullptr, h.obj));
 }
 
 // Any hit closer than `dis` (shadow test, RayTracer.cpp:510-520).

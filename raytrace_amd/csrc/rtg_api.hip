@@ -131,7 +131,7 @@ void Workspace::release() {
     b_reqs.release(), b_scenes.release(), b_chunks.release();
     for (int k = 0; k < kLevels; ++k) {
         w_rays[k].release(), w_hits[k].release(), w_nodes[k].release();
-        w_masks[k].release(), w_slots[k].release(), w_shadow[k].release();
+        w_masks[k].release(), w_slots[k].release(), w_shadow[k].release(), w_cs[k].release();
     }
     if (qcount_host) (void)hipHostFree(qcount_host);
     if (staging) (void)hipHostFree(staging);
@@ -306,8 +306,14 @@ int rtg_scene_update(rtg_scene_t scene, const rtg_scene_desc *desc) {
     if (!arena()) return fail(RTG_E_STATE, "call rtg_init first");
     Arena &A = *arena();
     std::lock_guard<std::mutex> l(A.lock);
-    // Always rebuild: shading tables are tiny and the BVH build is fast; the
-    // geometry hash only records the content for callers that compare it.
+    // Same geometry (objects, parts, triangles): only materials, lights,
+    // textures and the camera can have changed, so the BVH stays.
+    const uint64_t gh = geometry_hash(*desc);
+    SceneRes &R0 = scene->res;
+    if (R0.geom_hash != 0 && gh == R0.geom_hash) {
+        ++R0.shading_updates;
+        return scene_update_shading(R0, *desc, A.stream);
+    }
     SceneRes fresh;
     fresh.device = A.device;
     st = scene_build(fresh, *desc, A.stream);
@@ -333,7 +339,9 @@ int rtg_scene_update(rtg_scene_t scene, const rtg_scene_desc *desc) {
     R.bytes = fresh.bytes;
     R.n_nodes = fresh.n_nodes;
     R.lights_on = fresh.lights_on;
-    R.geom_hash = geometry_hash(*desc);
+    R.max_fanout = fresh.max_fanout;
+    R.geom_hash = gh;
+    ++R.rebuilds;
     return RTG_OK;
 }
 

@@ -175,6 +175,30 @@ int upload(DevBuf &b, const std::vector<T> &v, hipStream_t s) {
 
 }  // namespace
 
+// Lights switched on and the most child rays one hit can spawn (reflection
+// and/or refraction) over the materials of shown objects: the latter bounds
+// the wavefront queues per level.
+static void shading_state(SceneRes &S, const rtg_scene_desc &d) {
+    S.lights_on = 0;
+    for (int i = 0; i < d.n_lights; ++i)
+        if (d.lights[i].on) S.lights_on |= 1u << i;
+    auto fan_of = [&](int m) {
+        if (m < 0 || m >= d.n_materials) return 0;
+        return (d.materials[m].reflect > 0.01f ? 1 : 0) + (d.materials[m].refract > 0.01f ? 1 : 0);
+    };
+    S.max_fanout = 0;
+    for (int i = 0; i < d.n_objects; ++i) {
+        const rtg_object &o = d.objects[i];
+        if (!o.show) continue;
+        if (o.type == RTG_OBJ_MODEL) {
+            for (int k = 0; k < o.part_count; ++k) S.max_fanout = std::max(S.max_fanout, fan_of(d.parts[o.part_begin + k].mtl));
+        } else {
+            S.max_fanout = std::max(S.max_fanout, fan_of(o.mtl));
+        }
+    }
+
+}
+
 int scene_build(SceneRes &S, const rtg_scene_desc &d, hipStream_t stream) {
     ScenePrep prep;
     const std::string err = prepare_scene(d, prep);
@@ -295,25 +319,7 @@ int scene_build(SceneRes &S, const rtg_scene_desc &d, hipStream_t stream) {
     (void)hipEventDestroy(e1);
     S.build_ms = ms;
     S.n_nodes = n_emit;
-    S.lights_on = 0;
-    for (int i = 0; i < d.n_lights; ++i)
-        if (d.lights[i].on) S.lights_on |= 1u << i;
-    // Most child rays one hit can spawn (reflection and/or refraction) over
-    // the materials of shown objects: bounds the wavefront queues per level.
-    auto fan_of = [&](int m) {
-        if (m < 0 || m >= d.n_materials) return 0;
-        return (d.materials[m].reflect > 0.01f ? 1 : 0) + (d.materials[m].refract > 0.01f ? 1 : 0);
-    };
-    S.max_fanout = 0;
-    for (int i = 0; i < d.n_objects; ++i) {
-        const rtg_object &o = d.objects[i];
-        if (!o.show) continue;
-        if (o.type == RTG_OBJ_MODEL) {
-            for (int k = 0; k < o.part_count; ++k) S.max_fanout = std::max(S.max_fanout, fan_of(d.parts[o.part_begin + k].mtl));
-        } else {
-            S.max_fanout = std::max(S.max_fanout, fan_of(o.mtl));
-        }
-    }
+    shading_state(S, d);
 
     SceneView &v = S.view;
     v = SceneView{};
@@ -344,6 +350,38 @@ int scene_build(SceneRes &S, const rtg_scene_desc &d, hipStream_t stream) {
     S.bytes = S.nodes.bytes + S.prims.bytes + S.objs.bytes + S.brute.bytes + S.parts.bytes + S.tri_nrm.bytes +
               S.tri_uv.bytes + S.tri_pos.bytes + S.tri_part.bytes + S.tri_bins.bytes + S.mtls.bytes + S.lights.bytes + S.texs.bytes +
               S.texels.bytes;
+    return RTG_OK;
+}
+
+// Scene update that leaves the geometry unchanged (materials, lights,
+// textures, environment light, camera): re-upload those tables only; the
+// BVH, primitives and per-triangle data stay.
+int scene_update_shading(SceneRes &S, const rtg_scene_desc &d, hipStream_t stream) {
+    ScenePrep prep;
+    const std::string err = prepare_scene(d, prep);
+    if (!err.empty()) return fail(RTG_E_INVALID, err);
+    int st;
+    if ((st = upload(S.objs, prep.objs, stream)) || (st = upload(S.mtls, prep.mtls, stream)) ||
+        (st = upload(S.lights, prep.lights, stream)) || (st = upload(S.texs, prep.texs, stream)) ||
+        (st = upload(S.texels, prep.texels, stream)))
+        return st;
+    RTG_HIP(hipStreamSynchronize(stream));  // renders on other streams see the new tables
+    shading_state(S, d);
+    SceneView &v = S.view;
+    v.lights_on = S.lights_on;
+    v.objs = S.objs.as<DevObject>();
+    v.mtls = S.mtls.as<DevMaterial>();
+    v.lights = S.lights.as<DevLight>();
+    v.n_lights = d.n_lights;
+    v.n_texs = d.n_textures;
+    v.texs = S.texs.as<DevTexture>();
+    v.texels = S.texels.as<uint8_t>();
+    v.env = prep.env;
+    v.cam = prep.cam;
+    S.camera = d.camera;
+    S.bytes = S.nodes.bytes + S.prims.bytes + S.objs.bytes + S.brute.bytes + S.parts.bytes + S.tri_nrm.bytes +
+              S.tri_uv.bytes + S.tri_pos.bytes + S.tri_part.bytes + S.tri_bins.bytes + S.mtls.bytes + S.lights.bytes +
+              S.texs.bytes + S.texels.bytes;
     return RTG_OK;
 }
 

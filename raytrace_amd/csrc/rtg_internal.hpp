@@ -48,6 +48,8 @@ struct SceneRes {
     int n_nodes = 0;
     uint32_t lights_on = 0;  // bit i: light i is switched on
     int max_fanout = 2;      // most child rays per hit over the shown objects' materials
+    uint64_t rebuilds = 0;         // rtg_scene_update calls that rebuilt the BVH
+    uint64_t shading_updates = 0;  // rtg_scene_update calls that only re-uploaded shading tables
 };
 
 // Device memory of one in-flight render (or batch): traversal spill stacks,
@@ -58,8 +60,8 @@ struct Workspace {
     static constexpr int kLevels = RTG_MAX_LEVEL + 1;
     DevBuf spill, tables;
     uint64_t tables_key = 0;
-    DevBuf w_rays[kLevels], w_hits[kLevels], w_nodes[kLevels], w_masks[kLevels], w_slots[kLevels],
-        w_shadow[kLevels], w_samples, w_qcount;
+    DevBuf w_rays[kLevels], w_hits[kLevels], w_cs[kLevels], w_nodes[kLevels], w_masks[kLevels],
+        w_slots[kLevels], w_shadow[kLevels], w_samples, w_qcount;
     DevBuf b_reqs, b_scenes, b_chunks;  // batch: request table, scene table, chunk -> request
     unsigned *qcount_host = nullptr;    // pinned
     void *staging = nullptr;            // pinned upload staging (batch tables)
@@ -151,6 +153,8 @@ Arena *arena();
 int ensure(DevBuf &b, size_t bytes);
 
 int scene_build(SceneRes &S, const rtg_scene_desc &d, hipStream_t stream);
+// Update of a scene whose geometry is unchanged: shading tables only.
+int scene_update_shading(SceneRes &S, const rtg_scene_desc &d, hipStream_t stream);
 // Frame size of a render (params or the scene camera's) and parameter checks.
 int resolve_size(const SceneRes &S, const rtg_params *p, int &W, int &H);
 

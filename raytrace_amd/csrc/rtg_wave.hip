@@ -40,7 +40,8 @@ struct WRay {  // 48 B
 };
 struct WHit {  // 32 B
     float t, w, u, v;
-    int kind, obj, prim, pad;
+    int kind, obj, prim;
+    int self;  // 1: a self re-hit whose children take the identity stored in cs[i] (child_src)
 };
 struct WNode {  // 64 B
     float px, py, pz, nx, ny, nz, dx, dy, dz, vr, vg, vb, t;
@@ -195,7 +196,8 @@ __device__ inline int sample_req(const WaveArgs &A, int i) {
 // the plain closest-hit walk keeps its registers).
 template <bool B, bool R>
 __global__ __launch_bounds__(kBlockThreads, 6) void k_extend(SceneView S0, WaveArgs A, int k, const WRay *rays,
-                                                             const unsigned *n_ptr, WHit *hits, LineCtr *sched) {
+                                                             const unsigned *n_ptr, WHit *hits, int4 *cs,
+                                                             LineCtr *sched) {
     __shared__ int lds[kLdsStack * kBlockThreads];
     DevStack st = make_stack(lds, A.spill);
     const int n = k == 0 ? A.n_samples : (int)*n_ptr;
@@ -226,27 +228,41 @@ __global__ __launch_bounds__(kBlockThreads, 6) void k_extend(SceneView S0, WaveA
             q = load_ray(w, bwc, mtlrfr);
             if (q.src.obj >= 0) q.self_test = self_testable(S, q.r, q.src, kMiss);
         }
-        Hit h{kMiss, HIT_NONE, -1, -1, 0, 0, 0};
-        if (valid) {
-            if constexpr (R)
-                h = Rq.retest ? trace_closest<true>(S, q, st, &n_retest) : trace_closest<false>(S, q, st);
-            else
-                h = trace_closest(S, q, st);
-            if (k == 0)
-                ++c_prim;
-            else if (q.type == RAY_REFLECT)
-                ++c_refl;
-            else
-                ++c_refr;
-        }
-        hits[i] = WHit{h.t, h.w, h.u, h.v, h.kind, h.obj, h.prim, 0};
-        if (k == 0 && valid && si == 0) {
-            const size_t pix = (size_t)py * Rq.W + px;
-            if (Rq.ids) {
-                Rq.ids[2 * pix] = h.obj;
-                Rq.ids[2 * pix + 1] = id_prim(S, h);
+        // Pass 0 is the closest hit.  A self re-hit (child_src) takes a second
+        // pass over the objects before the source object; the plain kernel
+        // runs it through the same walk (one inlined copy of the traversal),
+        // after the hit record is stored so that little state stays live.
+        int before = 0x7fffffff;
+        for (;;) {
+            Hit h{kMiss, HIT_NONE, -1, -1, 0, 0, 0};
+            if (valid) {
+                if constexpr (R)
+                    h = Rq.retest ? trace_closest<true>(S, q, st, &n_retest) : trace_closest<false>(S, q, st);
+                else
+                    h = trace_closest<false, true>(S, q, st, nullptr, before);
             }
-            if (Rq.tprim) Rq.tprim[pix] = h.t;
+            if (before != 0x7fffffff) {
+                const SrcId c = child_src_of(S, q, h);
+                cs[i] = int4{c.obj, c.prim, c.bin, 0};
+                break;
+            }
+            if (valid) {
+                if (k == 0)
+                    ++c_prim;
+                else if (q.type == RAY_REFLECT)
+                    ++c_refl;
+                else
+                    ++c_refr;
+            }
+            const bool self = valid && h.kind != HIT_NONE && is_self_rehit(q, h);
+            hits[i] = WHit{h.t, h.w, h.u, h.v, h.kind, h.obj, h.prim, self ? 1 : 0};
+            if (!self) break;
+            if constexpr (R) {
+                const SrcId c = child_src(S, q, h, st);
+                cs[i] = int4{c.obj, c.prim, c.bin, 0};
+                break;
+            }
+            before = h.obj;
         }
     }
     add_counts(A.counts, c_prim, 0, c_refl, c_refr);
@@ -258,7 +274,8 @@ __global__ __launch_bounds__(kBlockThreads, 6) void k_extend(SceneView S0, WaveA
 
 template <bool B>
 __global__ __launch_bounds__(kBlockThreads, 4) void k_shade(SceneView S0, WaveArgs A, int k, const WRay *rays,
-                                                         const unsigned *n_ptr, const WHit *hits, WNode *nodes,
+                                                         const unsigned *n_ptr, const WHit *hits, const int4 *cs,
+                                                         WNode *nodes,
                                                          unsigned *masks, WRay *next, LevelCtr *ctr,
                                                          unsigned *shadow) {
     const int n = k == 0 ? A.n_samples : (int)*n_ptr;
@@ -291,6 +308,15 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_shade(SceneView S0, WaveAr
                 int px, py, si;
                 if (valid) valid = sample_pixel(R, i - R.sample_base, px, py, si);
                 if (valid) q = primary_ray_w(S, R, px, py, si);
+                if (valid && si == 0 && (R.ids || R.tprim)) {  // primary hit of the pixel's first sample
+                    const WHit wp = hits[i];
+                    const size_t pix = (size_t)py * R.W + px;
+                    if (R.ids) {
+                        R.ids[2 * pix] = wp.obj;
+                        R.ids[2 * pix + 1] = id_prim(S, Hit{wp.t, wp.kind, wp.obj, wp.prim, wp.w, wp.u, wp.v});
+                    }
+                    if (R.tprim) R.tprim[pix] = wp.t;
+                }
             } else {
                 q = load_ray(w, bwc, mtlrfr);
             }
@@ -300,7 +326,11 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_shade(SceneView S0, WaveAr
             nd.bits = 0;
             if (valid && wh.kind != HIT_NONE && !(wh.t > S.cam.zFar || wh.t < zNear)) {
                 const Hit h{wh.t, wh.kind, wh.obj, wh.prim, wh.w, wh.u, wh.v};
-                const HitRec hr = make_hitrec(S, q, h);
+                HitRec hr = make_hitrec(S, q, h);
+                if (wh.self) {
+                    const int4 c = cs[i];
+                    hr.id = SrcId{c.x, c.y, c.z};
+                }
                 const V3 vc = texel(S, hr.tex, hr.tu, hr.tv);
                 const DevMaterial &m = S.mtls[hr.mtl];
                 const float m_refract = R.no_refract ? 0.0f : m.refract;
@@ -523,7 +553,7 @@ struct WavePlan {
 };
 
 static size_t per_ray_bytes(int n_on) {
-    return sizeof(WRay) + sizeof(WHit) + sizeof(WNode) + sizeof(unsigned) + 2 * sizeof(float4) +
+    return sizeof(WRay) + sizeof(WHit) + sizeof(int4) + sizeof(WNode) + sizeof(unsigned) + 2 * sizeof(float4) +
            (size_t)n_on * sizeof(unsigned);
 }
 
@@ -555,7 +585,8 @@ static WavePlan make_plan(int n_samples, int max_level, int fanout, int n_on, in
 static int reserve_level(Workspace &ws, const WavePlan &pl, int k, size_t n_k) {
     const size_t cap_next = k < pl.max_level ? (size_t)std::max(1, pl.fanout) * n_k : 1;
     const int kn = k + 1 < Workspace::kLevels ? k + 1 : k;
-    if (ws.grow(ws.w_hits[k], n_k * sizeof(WHit)) || ws.grow(ws.w_nodes[k], n_k * sizeof(WNode)) ||
+    if (ws.grow(ws.w_hits[k], n_k * sizeof(WHit)) || ws.grow(ws.w_cs[k], n_k * sizeof(int4)) ||
+        ws.grow(ws.w_nodes[k], n_k * sizeof(WNode)) ||
         ws.grow(ws.w_masks[k], n_k * sizeof(unsigned)) || ws.grow(ws.w_slots[k], 2 * n_k * sizeof(float4)) ||
         ws.grow(ws.w_shadow[k], std::max<size_t>(1, n_k * pl.n_on) * sizeof(unsigned)) ||
         ws.grow(ws.w_rays[kn], cap_next * sizeof(WRay)))
@@ -615,14 +646,14 @@ static int enqueue_wave(const SceneView &S0, const WaveArgs &a, const WavePlan &
         int kt = ktime_begin(A, stream, KT_EXTEND, n_k);
         if (retest)
             k_extend<B, true><<<grid(g.extend, n_k), kBlockThreads, 0, stream>>>(
-                S0, args, k, rays, n_ptr, ws.w_hits[k].as<WHit>(), ctr[k].sched[0]);
+                S0, args, k, rays, n_ptr, ws.w_hits[k].as<WHit>(), ws.w_cs[k].as<int4>(), ctr[k].sched[0]);
         else
             k_extend<B, false><<<grid(g.extend, n_k), kBlockThreads, 0, stream>>>(
-                S0, args, k, rays, n_ptr, ws.w_hits[k].as<WHit>(), ctr[k].sched[0]);
+                S0, args, k, rays, n_ptr, ws.w_hits[k].as<WHit>(), ws.w_cs[k].as<int4>(), ctr[k].sched[0]);
         ktime_end(A, stream, kt, n_ptr);
         kt = ktime_begin(A, stream, KT_SHADE, n_k);
         k_shade<B><<<grid(g.shade, n_k), kBlockThreads, 0, stream>>>(
-            S0, args, k, rays, n_ptr, ws.w_hits[k].as<WHit>(), ws.w_nodes[k].as<WNode>(),
+            S0, args, k, rays, n_ptr, ws.w_hits[k].as<WHit>(), ws.w_cs[k].as<int4>(), ws.w_nodes[k].as<WNode>(),
             ws.w_masks[k].as<unsigned>(), next, ctr + k, ws.w_shadow[k].as<unsigned>());
         ktime_end(A, stream, kt, n_ptr);
         k_fill_slots<<<256, 256, 0, stream>>>(ws.w_slots[k].as<float4>(), n_ptr, k == 0 ? pl.n_samples : -1);

@@ -24,9 +24,15 @@ namespace {
 // Traversal stack that records its high-water mark (the device stack holds
 // kLdsStack entries in LDS and kStackMax in total).
 std::atomic<int> g_max_sp{0};
+std::atomic<uint64_t> g_nodes{0}, g_prims{0};
+
 struct TrackStack {
     int d[256];
     int sp = 0;
+    uint64_t nodes = 0, prims = 0;  // BVH node visits and primitive tests of this thread
+    void on_node() { ++nodes; }
+    void on_prim() { ++prims; }
+    ~TrackStack() { g_nodes += nodes, g_prims += prims; }
     void reset() { sp = 0; }
     void push(int v) {
         d[sp++] = v;
@@ -246,6 +252,12 @@ int emul_render(const rtg_scene_desc *d, const rtg_params *p, int threads, const
 
 // High-water mark of the traversal stack over the last emul_render.
 int emul_max_stack(void) { return g_max_sp.load(); }
+
+// BVH node visits and primitive tests of the renders since the last call.
+void emul_traffic(uint64_t out[2]) {
+    out[0] = g_nodes.exchange(0);
+    out[1] = g_prims.exchange(0);
+}
 
 // Level-0 length the batch engine pads `total` samples to (rt_prep.hpp).
 unsigned long long emul_batch_bucket(unsigned long long total) { return (unsigned long long)rtg::batch_bucket(total); }

@@ -152,3 +152,23 @@ def test_debug_modes(mode):
         o, e = _run(hs, p)
         _assert_parity(o, e, exact=False)
         assert o["stats"].rays_refract == 0 and e["counts"][3] == 0
+
+
+def test_refracting_sphere_children_identity():
+    """Regression: a refraction ray that leaves a sphere hits that same sphere
+    again (the exit).  RTfrac hands its children and shadow rays `newobj`, the
+    last object that changed HitRes::obj during the scan, which for an exit is
+    the closest object scanned *before* the sphere (here often the plane), not
+    the sphere; the children then skip that object.  Using the sphere instead
+    changed 1,717 pixels of this scene."""
+    hs = rt.HostScene.workload(1)
+    hs.set_material(1, (0.1, 0.1, 0.1), (0.3, 0.4, 0.5), (0.5, 0.5, 0.5), 20.0, reflect=0.2, refract=0.6, rfr=1.4)
+    hs.set_material(4, (0.1, 0.1, 0.1), (0.6, 0.2, 0.2), (0.5, 0.5, 0.5), 20.0, reflect=0.0, refract=0.8, rfr=1.6)
+    b = hs.add_ballplane(0.4)  # BallPlane: 16 spheres, each re-hit from inside
+    hs.set_material(b, (0.1, 0.1, 0.1), (0.2, 0.6, 0.2), (0.5, 0.5, 0.5), 20.0, reflect=0.3, refract=0.5, rfr=1.3)
+    for ml in (2, 5):
+        p = rt.params(640, 480, max_level=ml)
+        o, e = _run(hs, p)
+        _assert_parity(o, e)
+        assert list(e["counts"]) == [o["stats"].rays_primary, o["stats"].rays_shadow, o["stats"].rays_reflect,
+                                     o["stats"].rays_refract]

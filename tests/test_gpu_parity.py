@@ -285,3 +285,34 @@ def test_debug_modes_on_device(rt_gpu, mode):
     buf, _ = cases[0][0].raytrace(mode, 3)
     img = buf[:640 * 480 * 3].reshape(480, 640, 3)
     assert (img[448:] == 127).all()
+
+
+def test_scene_update_paths(rt_gpu):
+    """rtg_scene_update (the reference's Scene mutations, Scene.cpp:157-301):
+    material, light and camera changes re-upload the shading tables only (the
+    BVH stays: same build time); moving an object rebuilds.  Each updated
+    scene renders exactly as a freshly created one, including a material that
+    turns refraction on (two child rays per hit: the queue bounds follow)."""
+    rt = rt_gpu
+    log("case: scene updates on workload 1: material (adds refraction), light, object move")
+    hs = rt.HostScene.workload(1)
+    ds = rt.DeviceScene(hs)
+    p = rt.params(640, 480, max_level=4)
+    b0 = ds.build_ms
+
+    def same_as_fresh():
+        g, f = ds.render(p), rt.DeviceScene(hs).render(p)
+        assert (g["rgb"] == f["rgb"]).all() and (g["ids"] == f["ids"]).all()
+        _check(compare_frames(g, _oracle(hs, p)))
+
+    hs.set_material(1, (0.1, 0.1, 0.1), (0.3, 0.4, 0.5), (0.5, 0.5, 0.5), 20.0, reflect=0.2, refract=0.6, rfr=1.4)
+    ds.update(hs)
+    assert ds.build_ms == b0  # shading tables only
+    same_as_fresh()
+    hs.set_light_position(0, (2.0, 9.0, 3.0, 1.0))
+    ds.update(hs)
+    assert ds.build_ms == b0
+    same_as_fresh()
+    hs.move_pos(2, 2, (0.5, 0.25, -0.5))  # MovePos(MY_MODEL_OBJECT, 2): geometry changed
+    ds.update(hs)
+    same_as_fresh()
