@@ -78,14 +78,31 @@ def bench_batch(args) -> dict:
     hs, ds = batch_scenes(rt)
     reqs = rt.gen_requests(256, 0.0, seed=3)
     items = [(ds[s], rt.params(w, h, max_level=hs[s].info.max_level)) for s, w, h, _ in reqs]
-    for _ in range(max(2, args.warmup)):
-        _, st = rt.render_batch(items, ids=False)
-    t0 = time.perf_counter()
-    replays = 0
-    for _ in range(args.steps):
-        _, st = rt.render_batch(items, ids=False)
-        replays += st.graph_replays
-    batch_s = (time.perf_counter() - t0) / args.steps
+
+    def timed_batches():
+        for _ in range(max(2, args.warmup)):
+            rt.render_batch(items, ids=False)
+        t0 = time.perf_counter()
+        replays = 0
+        for _ in range(args.steps):
+            _, st = rt.render_batch(items, ids=False)
+            replays += st.graph_replays
+        return (time.perf_counter() - t0) / args.steps, st, replays
+
+    default_tuning = rt.get_tuning()
+    tune = None
+    if not args.no_autotune:
+        # the tuner's runs also warm the GPU, so both timings below start warm
+        t0 = time.perf_counter()
+        best, trials = rt.autotune(items, reps=3)
+        tune_s = time.perf_counter() - t0
+        rt.set_tuning(**default_tuning)
+        default_s, _, _ = timed_batches()
+        rt.set_tuning(**best)
+        tune = {"best": best, "default": default_tuning, "seconds": round(tune_s, 2),
+                "trials": [{**t, "device_ms": round(ms, 3)} for t, ms in trials],
+                "default_ms_per_batch": round(default_s * 1e3, 3)}
+    batch_s, st, replays = timed_batches()
     rays = st.rays_total
     for d, p in items[:8]:
         d.render(p, ids=False, tprim=False)  # warm the single-frame path
@@ -103,7 +120,9 @@ def bench_batch(args) -> dict:
         "config": {"workload": "256 requests 64x64..1024x768 over 8 resident scenes (1k..1M triangles, 1-8 lights, "
                                "depth 2-6), RGB to host", "rays_per_batch": rays,
                    "pixels_per_batch": sum(p.width * p.height for _, p in items),
-                   "scene_triangles": [int(h.info.n_tris) for h in hs], "graph_replayed_steps": replays},
+                   "scene_triangles": [int(h.info.n_tris) for h in hs], "graph_replayed_steps": replays,
+                   "tuning": rt.get_tuning()},
+        "autotune": tune,
         "plain": {"ms_per_batch": round(plain_s * 1e3, 3), "mrays_per_s": round(rays / plain_s / 1e6, 2),
                   "what": "the same 256 requests, one rtg_render each"},
         "speedup_vs_plain": round(plain_s / batch_s, 2),
@@ -226,6 +245,8 @@ def main() -> int:
     ap.add_argument("--rates", default="50,100,200,500,1000,2000", help="workload 4: offered rates (requests/s)")
     ap.add_argument("--duration", type=float, default=2.0, help="workload 4: seconds of arrivals per rate")
     ap.add_argument("--slots", type=int, default=2, help="workload 4: concurrent batches")
+    ap.add_argument("--no-autotune", action="store_true", help="workload 3: keep the default launch tuning")
+    ap.add_argument("--autotune", action="store_true", help="workload 2: autotune the launch tuning first")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (reference for FETCH_SIZE / WRITE_SIZE scaling)")
     args = ap.parse_args()
@@ -280,6 +301,12 @@ def main() -> int:
         if rc != 0:
             raise RuntimeError(f"rtg_render_device failed: {_capi.last_error()}")
 
+    tune = None
+    if args.autotune:
+        default_tuning = rt.get_tuning()
+        best, trials = rt.autotune([(ds, p)], reps=3)
+        tune = {"best": best, "default": default_tuning,
+                "trials": [{**t, "device_ms": round(ms, 3)} for t, ms in trials]}
     for _ in range(args.warmup):
         step()
     stream.synchronize()
@@ -361,9 +388,11 @@ def main() -> int:
         "config": {"workload": "1920x1080 1spp depth 4, 200k-tri displaced icosphere + 64 spheres, 4 point lights",
                    "engine": {"mega": "megakernel", "wave": "wavefront"}[eng_name], "rays_per_frame": rays_per_frame,
                    "rays_by_type": dict(zip(["primary", "shadow", "reflect", "refract"], per_type)),
-                   "bvh_build_ms": round(build_ms, 3)},
+                   "bvh_build_ms": round(build_ms, 3), "tuning": rt.get_tuning()},
         "roofline": roofline, "cpu_baseline": cpu,
     }
+    if tune is not None:
+        line["autotune"] = tune
     print(json.dumps(line))
     return 0
 

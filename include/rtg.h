@@ -283,6 +283,41 @@ typedef struct rtg_request {
  * counterpart in the reference, which renders one frame per start(). */
 int rtg_render_batch(const rtg_request *reqs, int n, rtg_stats *stats);
 
+/* ---- launch tuning of the wavefront engine ----------------------------- */
+
+/* Tile and queue sizes of the wavefront engine (frames and batches).  None of
+ * them changes any pixel: only the order in which rays are traced and how the
+ * persistent kernels split the queues.  No counterpart in the reference,
+ * whose CPU workers take fixed 64x64 tiles (RayTracer.cpp:13-41). */
+typedef struct rtg_tuning {
+    int32_t tile_w;   /* width of the pixel block whose 64 primary rays one wave traces:
+                         4, 8, 16 or 32 (height 64 / tile_w); default 8 */
+    int32_t grab;     /* 64-item queue chunks a wave takes per atomic: 1, 2 or 4; default 1 */
+    int32_t grid_pct; /* persistent grid of the traversal kernels, percent of the blocks
+                         the GPU holds at once: 25..100; default 100 */
+    int32_t reserved; /* 0 */
+} rtg_tuning;
+
+/* One timed candidate of rtg_autotune. */
+typedef struct rtg_tune_trial {
+    rtg_tuning tuning;
+    double device_ms; /* median device time of the requests under this tuning */
+} rtg_tune_trial;
+
+/* The tuning in use (rtg_init installs the defaults). */
+int rtg_get_tuning(rtg_tuning *out);
+/* Install a tuning; RTG_E_INVALID for a value outside the ranges above. */
+int rtg_set_tuning(const rtg_tuning *t);
+/* Autotuner: times the wavefront engine on `reqs` (one request: the frame
+ * path; several: one batch, as rtg_render_batch renders it) under candidate
+ * tunings, by coordinate descent over tile_w, grab and grid_pct from the
+ * tuning in use, each candidate the median of `reps` timed runs after one
+ * warmup run.  Installs the fastest and writes it to `best`; the timed
+ * candidates go to `trials` (up to max_trials; *n_trials = how many were
+ * timed).  Outputs stay on the device. */
+int rtg_autotune(const rtg_request *reqs, int n, int reps, rtg_tuning *best, rtg_tune_trial *trials,
+                 int max_trials, int *n_trials);
+
 /* ---- serving loop ----------------------------------------------------- */
 
 /* One request of a serving run: a frame of a resident scene, arriving at

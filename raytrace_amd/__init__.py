@@ -225,6 +225,44 @@ def render_batch(items, ids: bool = True, eager: bool = False):
     return outs, st
 
 
+def _tuning_dict(t) -> dict:
+    return {"tile_w": t.tile_w, "grab": t.grab, "grid_pct": t.grid_pct}
+
+
+def get_tuning() -> dict:
+    """Launch tuning of the wavefront engine in use: {tile_w, grab, grid_pct}."""
+    lib = _capi.load()
+    t = _capi.rtg_tuning()
+    check(lib.rtg_get_tuning(C.byref(t)), "rtg_get_tuning")
+    return _tuning_dict(t)
+
+
+def set_tuning(tile_w: int = 8, grab: int = 1, grid_pct: int = 100) -> None:
+    """Install a launch tuning (rtg_set_tuning); pixels do not depend on it."""
+    lib = _capi.load()
+    t = _capi.rtg_tuning(tile_w=tile_w, grab=grab, grid_pct=grid_pct)
+    check(lib.rtg_set_tuning(C.byref(t)), "rtg_set_tuning")
+
+
+def autotune(items, reps: int = 3):
+    """rtg_autotune over [(DeviceScene, rtg_params), ...] (one item: the frame
+    path; several: one batch).  Installs the fastest tuning and returns
+    (best {tile_w, grab, grid_pct}, [(tuning dict, median device ms), ...])."""
+    lib = _capi.load()
+    n = len(items)
+    reqs = (_capi.rtg_request * max(1, n))()
+    for k, (ds, p) in enumerate(items):
+        reqs[k].scene = ds.h
+        reqs[k].params = p
+    best = _capi.rtg_tuning()
+    cap = 64
+    trials = (_capi.rtg_tune_trial * cap)()
+    nt = C.c_int()
+    check(lib.rtg_autotune(reqs, n, reps, C.byref(best), trials, cap, C.byref(nt)), "rtg_autotune")
+    return _tuning_dict(best), [(_tuning_dict(trials[i].tuning), trials[i].device_ms)
+                                for i in range(min(nt.value, cap))]
+
+
 def gen_requests(n: int, rate: float = 0.0, seed: int = 3):
     """Request stream of workloads 3/4: [(scene index, width, height, arrival_s)]."""
     lib = _capi.load()

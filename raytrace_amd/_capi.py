@@ -163,6 +163,14 @@ class rtg_request(C.Structure):
     ]
 
 
+class rtg_tuning(C.Structure):
+    _fields_ = [("tile_w", C.c_int32), ("grab", C.c_int32), ("grid_pct", C.c_int32), ("reserved", C.c_int32)]
+
+
+class rtg_tune_trial(C.Structure):
+    _fields_ = [("tuning", rtg_tuning), ("device_ms", C.c_double)]
+
+
 _P = C.c_void_p
 _I = C.c_int
 _F = C.c_float
@@ -188,6 +196,10 @@ SIGNATURES = {
     "rtg_job_free": (_I, [_P]),
     "rtg_render_batch": (_I, [C.POINTER(rtg_request), _I, C.POINTER(rtg_stats)]),
     "rtg_kernel_times": (_I, [C.POINTER(rtg_kernel_time), _I, C.POINTER(_I)]),
+    "rtg_get_tuning": (_I, [C.POINTER(rtg_tuning)]),
+    "rtg_set_tuning": (_I, [C.POINTER(rtg_tuning)]),
+    "rtg_autotune": (_I, [C.POINTER(rtg_request), _I, _I, C.POINTER(rtg_tuning), C.POINTER(rtg_tune_trial), _I,
+                          C.POINTER(_I)]),
     "rtg_serve": (_I, [C.POINTER(rtg_serve_request), _I, C.POINTER(rtg_serve_opts), C.POINTER(rtg_serve_stats)]),
     # rtg_host.h
     "rtg_host_scene_config": (_I, [_I, C.POINTER(rtg_gen_opts), C.POINTER(_P), C.POINTER(rtg_gen_info)]),

@@ -62,29 +62,36 @@ struct alignas(256) LineCtr {
     unsigned pad[63];
 };
 
-// Per-wave dynamic work split: a wave takes chunks of 64 items, first from its
-// XCD's contiguous eighth of the range (one counter per XCD, so neighbouring
-// chunks share an L2), then from the other XCDs' eighths once its own is
-// exhausted.  `ctr` points to 8 counters that are zero at launch.  Every lane
-// of the wave must call next(); it returns the chunk's first item, or -1
-// when every chunk is taken.  A region whose counter was seen past its end is
-// never tried again by the wave, so a wave makes at most 8 failed attempts.
+// Per-wave dynamic work split: a wave grabs chunks of 64 << grab_lg items,
+// first from its XCD's contiguous eighth of the range (one counter per XCD, so
+// neighbouring chunks share an L2), then from the other XCDs' eighths once its
+// own is exhausted, and hands a grabbed chunk out 64 items per next() call.
+// `ctr` points to 8 counters that are zero at launch.  Every lane of the wave
+// must call next(); it returns the first item of the next 64, or -1 when every
+// chunk is taken.  A region whose counter was seen past its end is never tried
+// again by the wave, so a wave makes at most 8 failed grabs.
 struct WaveQueue {
     LineCtr *ctr;
-    int nchunks, region, x;
+    int n, glg, nchunks, region, x;
+    int cur, end;   // the grabbed chunk: next item handed out, end (wave-uniform)
     uint32_t dead;  // regions this wave has seen exhausted
-    __device__ WaveQueue(LineCtr *c, int n_items)
-        : ctr(c), nchunks((n_items + 63) >> 6), region((((n_items + 63) >> 6) + 7) >> 3), x(blockIdx.x & 7),
-          dead(0) {}
+    __device__ WaveQueue(LineCtr *c, int n_items, int grab_lg = 0)
+        : ctr(c), n(n_items), glg(grab_lg), nchunks((n_items + (64 << grab_lg) - 1) >> (6 + grab_lg)),
+          region((nchunks + 7) >> 3), x(blockIdx.x & 7), cur(0), end(0), dead(0) {}
     __device__ int region_len(int r) const { return max(0, min(region, nchunks - r * region)); }
     __device__ int next() {
+        if (cur + 64 < end) return cur += 64;
         if (nchunks == 0) return -1;
         const int lane = __lane_id();
         for (;;) {
             int c = 0;
             if (lane == 0) c = (int)atomicAdd(&ctr[x].v, 1u);
             c = __shfl(c, 0, 64);
-            if (c < region_len(x)) return (x * region + c) << 6;
+            if (c < region_len(x)) {
+                cur = (x * region + c) << (6 + glg);
+                end = min(n, cur + (64 << glg));
+                return cur;
+            }
             dead |= 1u << x;
             // regions with chunks left, read in one round trip
             bool left = false;

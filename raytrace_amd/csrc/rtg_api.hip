@@ -583,4 +583,149 @@ int rtg_render_batch(const rtg_request *reqs, int n, rtg_stats *stats) {
     return RTG_OK;
 }
 
+// ---- launch tuning ------------------------------------------------------
+
+static int tuning_from(const rtg_tuning &t, Tuning &out) {
+    const int lg = t.tile_w == 4 ? 2 : t.tile_w == 8 ? 3 : t.tile_w == 16 ? 4 : t.tile_w == 32 ? 5 : -1;
+    const int g = t.grab == 1 ? 0 : t.grab == 2 ? 1 : t.grab == 4 ? 2 : -1;
+    if (lg < 0) return fail(RTG_E_INVALID, "tuning: tile_w must be 4, 8, 16 or 32");
+    if (g < 0) return fail(RTG_E_INVALID, "tuning: grab must be 1, 2 or 4");
+    if (t.grid_pct < 25 || t.grid_pct > 100) return fail(RTG_E_INVALID, "tuning: grid_pct must be 25..100");
+    out.tile_lg = lg, out.grab_lg = g, out.grid_pct = t.grid_pct;
+    return RTG_OK;
+}
+
+static rtg_tuning tuning_to(const Tuning &t) { return rtg_tuning{1 << t.tile_lg, 1 << t.grab_lg, t.grid_pct, 0}; }
+
+int rtg_get_tuning(rtg_tuning *out) {
+    if (out == nullptr) return fail(RTG_E_INVALID, "NULL argument");
+    if (!arena()) return fail(RTG_E_STATE, "call rtg_init first");
+    Arena &A = *arena();
+    std::lock_guard<std::mutex> l(A.lock);
+    *out = tuning_to(A.tune);
+    return RTG_OK;
+}
+
+int rtg_set_tuning(const rtg_tuning *t) {
+    if (t == nullptr) return fail(RTG_E_INVALID, "NULL argument");
+    Tuning v;
+    const int st = tuning_from(*t, v);
+    if (st != RTG_OK) return st;
+    if (!arena()) return fail(RTG_E_STATE, "call rtg_init first");
+    Arena &A = *arena();
+    std::lock_guard<std::mutex> l(A.lock);
+    A.tune = v;
+    return RTG_OK;
+}
+
+int rtg_autotune(const rtg_request *reqs, int n, int reps, rtg_tuning *best, rtg_tune_trial *trials, int max_trials,
+                 int *n_trials) {
+    if (n < 1 || reqs == nullptr) return fail(RTG_E_INVALID, "autotune needs at least one request");
+    if (n > 4096) return fail(RTG_E_LIMIT, "more than 4096 requests in a batch");
+    if (reps < 1 || reps > 100) return fail(RTG_E_INVALID, "reps must be 1..100");
+    if (max_trials < 0 || (max_trials > 0 && trials == nullptr)) return fail(RTG_E_INVALID, "bad trial buffer");
+    if (!arena()) return fail(RTG_E_STATE, "call rtg_init first");
+    Arena &A = *arena();
+    std::vector<BatchItem> items(n);
+    std::vector<size_t> off_rgb(n);
+    size_t rgb_bytes = 0;
+    for (int i = 0; i < n; ++i) {
+        const rtg_request &q = reqs[i];
+        if (q.scene == nullptr) return fail(RTG_E_INVALID, "request without scene");
+        int W, H;
+        const int st = resolve_size(q.scene->res, &q.params, W, H);
+        if (st != RTG_OK) return st;
+        if (q.params.mode != RTG_MODE_RAYTRACE && q.params.mode != RTG_MODE_REFRACTTEST &&
+            q.params.mode != RTG_MODE_REFLECTTEST && q.params.mode != 0)
+            return fail(RTG_E_INVALID, "the wavefront engine renders the ray-traced modes");
+        items[i] = BatchItem{&q.scene->res, q.params, W, H, nullptr, nullptr, nullptr};
+        off_rgb[i] = rgb_bytes;
+        rgb_bytes += ((size_t)W * H * 3 + 255) & ~size_t(255);
+    }
+    std::lock_guard<std::mutex> l(A.lock);
+    A.ktimes = false;
+    if (ensure(A.out_rgb, rgb_bytes)) return fail(RTG_E_NOMEM, "autotune output buffers");
+    for (int i = 0; i < n; ++i) items[i].rgb = A.out_rgb.as<uint8_t>() + off_rgb[i];
+    hipStream_t s = A.stream;
+    unsigned long long *counts = A.counters.as<unsigned long long>();
+    auto run = [&]() -> int {
+        k_zero_u64<<<1, 64, 0, s>>>(counts, 5);
+        int st = A.ws.order_before(s);
+        if (st != RTG_OK) return st;
+        if (n == 1) {
+            const RenderOut o{items[0].rgb, nullptr, nullptr, counts, counts + 4};
+            st = launch_wavefront(*items[0].scene, items[0].params, items[0].W, items[0].H, o, s, A.ws, A, nullptr);
+        } else {
+            st = launch_batch(items.data(), n, counts, counts + 4, s, A.ws, A, 0, nullptr);
+        }
+        const int st2 = A.ws.mark_done(s);
+        return st != RTG_OK ? st : st2;
+    };
+    // median device time of `reps` runs after one warmup run
+    auto time_of = [&](const Tuning &t, double &ms) -> int {
+        A.tune = t;
+        int st = run();
+        if (st != RTG_OK) return st;
+        std::vector<double> v;
+        for (int r = 0; r < reps; ++r) {
+            RTG_HIP(hipEventRecord(A.ev0, s));
+            if ((st = run()) != RTG_OK) return st;
+            RTG_HIP(hipEventRecord(A.ev1, s));
+            RTG_HIP(hipEventSynchronize(A.ev1));
+            float e = 0.0f;
+            RTG_HIP(hipEventElapsedTime(&e, A.ev0, A.ev1));
+            v.push_back(e);
+        }
+        std::sort(v.begin(), v.end());
+        ms = v[v.size() / 2];
+        return RTG_OK;
+    };
+    const Tuning start = A.tune;
+    std::vector<std::pair<uint64_t, double>> seen;  // tuning key -> median ms
+    int count = 0;
+    auto trial = [&](const Tuning &t, double &ms) -> int {
+        for (const auto &e : seen)
+            if (e.first == t.key()) {
+                ms = e.second;
+                return RTG_OK;
+            }
+        const int st = time_of(t, ms);
+        if (st != RTG_OK) return st;
+        seen.emplace_back(t.key(), ms);
+        if (count < max_trials) trials[count] = rtg_tune_trial{tuning_to(t), ms};
+        ++count;
+        return RTG_OK;
+    };
+    // coordinate descent: tile shape, then queue grab, then grid; repeated
+    // while a pass changes anything (at most three passes)
+    static const int tiles[] = {2, 3, 4, 5}, grabs[] = {0, 1, 2}, grids[] = {50, 75, 100};
+    Tuning cur = start;
+    double cur_ms = 0.0;
+    int st = trial(cur, cur_ms);
+    for (int pass = 0; st == RTG_OK && pass < 3; ++pass) {
+        bool changed = false;
+        for (int knob = 0; knob < 3 && st == RTG_OK; ++knob) {
+            const int *vals = knob == 0 ? tiles : knob == 1 ? grabs : grids;
+            const int nv = knob == 0 ? 4 : 3;
+            for (int k = 0; k < nv && st == RTG_OK; ++k) {
+                Tuning c = cur;
+                (knob == 0 ? c.tile_lg : knob == 1 ? c.grab_lg : c.grid_pct) = vals[k];
+                double ms = 0.0;
+                st = trial(c, ms);
+                if (st == RTG_OK && ms < cur_ms) {
+                    changed = changed || c.key() != cur.key();
+                    cur = c, cur_ms = ms;
+                }
+            }
+        }
+        if (!changed) break;
+    }
+    RTG_HIP(hipStreamSynchronize(s));
+    A.tune = st == RTG_OK ? cur : start;
+    if (st != RTG_OK) return st;
+    if (best) *best = tuning_to(cur);
+    if (n_trials) *n_trials = count;
+    return RTG_OK;
+}
+
 }  // extern "C"

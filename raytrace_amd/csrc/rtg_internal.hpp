@@ -105,8 +105,18 @@ struct ServeSlot {
 
 // Per-device render state: default stream and workspace, outputs, cancel
 // flag, kernel timing, serving slots.
+// Launch tuning of the wavefront engine (rtg_tuning): pixel block of a wave
+// (tile_lg = log2 of its width, 2..5; height 64 >> tile_lg), 64-item queue
+// chunks per grab (1 << grab_lg, 0..2), persistent grid in percent of the
+// resident blocks (25..100).
+struct Tuning {
+    int tile_lg = 3, grab_lg = 0, grid_pct = 100;
+    uint64_t key() const { return (uint64_t)tile_lg | ((uint64_t)grab_lg << 4) | ((uint64_t)grid_pct << 8); }
+};
+
 struct Arena {
     static constexpr int kLevels = Workspace::kLevels;
+    Tuning tune;
     int device = -1;
     int cus = 0;
     // Largest queue memory (bytes) one wavefront may reserve to run without
@@ -197,7 +207,11 @@ int launch_batch(const BatchItem *items, int n, unsigned long long *counts, unsi
 // admission by memory budget; 0 when the bound exceeds `budget`.
 size_t batch_queue_bytes(size_t samples, int max_level, int fanout, int n_on, size_t budget);
 // Level-0 queue length of a single frame.
-inline size_t frame_samples(int W, int H, int spp) { return (size_t)((W + 7) / 8) * ((H + 7) / 8) * 64 * (spp > 0 ? spp : 1); }
+// Level-0 samples of a W x H frame rendered in pixel blocks of (1 << tile_lg) x (64 >> tile_lg).
+inline size_t frame_samples(int W, int H, int spp, int tile_lg) {
+    const int tw = 1 << tile_lg, th = 64 >> tile_lg;
+    return (size_t)((W + tw - 1) / tw) * ((H + th - 1) / th) * 64 * (spp > 0 ? spp : 1);
+}
 
 }  // namespace rtg
 

@@ -138,7 +138,7 @@ int serve_run(rtg_serve_request *reqs, int n, const rtg_serve_opts &o, rtg_serve
         if (q.params.mode != RTG_MODE_RAYTRACE && q.params.mode != RTG_MODE_REFRACTTEST &&
             q.params.mode != RTG_MODE_REFLECTTEST && q.params.mode != 0)
             return fail(RTG_E_INVALID, "the serving loop renders the ray-traced modes");
-        info[i].samples = frame_samples(info[i].W, info[i].H, q.params.spp);
+        info[i].samples = frame_samples(info[i].W, info[i].H, q.params.spp, A.tune.tile_lg);
         info[i].fanout = q.scene->res.max_fanout;
         info[i].n_on = __builtin_popcount(q.scene->res.lights_on);
     }

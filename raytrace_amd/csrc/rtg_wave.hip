@@ -67,7 +67,8 @@ struct WaveReq {
     int W, H, spp, max_level, tile_compat;
     int retest;       // RTG_FLAG_FP64_RETEST: FP64 near-tie re-test of closest hits
     int no_refract;   // the reflect test (RTflec, RayTracer.cpp:332-448): no refraction branch
-    int bw;           // 8x8 pixel blocks per row
+    int tile_lg;      // pixel blocks of (1 << tile_lg) x (64 >> tile_lg): one wave's primary rays
+    int bw;           // pixel blocks per row
     int row0;         // first row of pixel blocks this pass covers (a frame may take several passes)
     int n_samples;    // pixel blocks of the pass * 64 * spp
     int sample_base;  // first level-0 queue slot of the request
@@ -86,6 +87,7 @@ struct WaveArgs {
     const SceneView *scenes;  // B = true
     const int *chunk_req;     // B = true: request of each 64-sample chunk of level 0 (-1: padding)
     int n_samples;            // level-0 queue length
+    int grab_lg;              // WaveQueue grabs of 64 << grab_lg items
     unsigned long long *counts;
     unsigned long long *retests;  // closest-hit queries re-tested in FP64
     int *spill;
@@ -111,8 +113,9 @@ __device__ inline bool sample_pixel(const WaveReq &R, int s, int &px, int &py, i
     const int pb = s / per_block, r = s - pb * per_block;
     const int lane = r / R.spp;
     si = r - lane * R.spp;
-    px = (pb % R.bw) * 8 + (lane & 7);
-    py = (R.row0 + pb / R.bw) * 8 + (lane >> 3);
+    const int lg = R.tile_lg;
+    px = (pb % R.bw) * (1 << lg) + (lane & ((1 << lg) - 1));
+    py = (R.row0 + pb / R.bw) * (64 >> lg) + (lane >> lg);
     if (px >= R.W || py >= R.H) return false;
     if (R.tile_compat && (px >= ((R.W >> 6) << 6) || py >= ((R.H >> 6) << 6))) return false;
     return true;
@@ -201,7 +204,7 @@ __global__ __launch_bounds__(kBlockThreads, 6) void k_extend(SceneView S0, WaveA
     __shared__ int lds[kLdsStack * kBlockThreads];
     DevStack st = make_stack(lds, A.spill);
     const int n = k == 0 ? A.n_samples : (int)*n_ptr;
-    WaveQueue wq(sched, n);
+    WaveQueue wq(sched, n, A.grab_lg);
     uint32_t c_prim = 0, c_refl = 0, c_refr = 0;
     int n_retest = 0;
     for (int base = wq.next(); base >= 0; base = wq.next()) {
@@ -279,7 +282,7 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_shade(SceneView S0, WaveAr
                                                          unsigned *masks, WRay *next, LevelCtr *ctr,
                                                          unsigned *shadow) {
     const int n = k == 0 ? A.n_samples : (int)*n_ptr;
-    WaveQueue wq(ctr->sched[1], n);
+    WaveQueue wq(ctr->sched[1], n, A.grab_lg);
     uint32_t c_shadow = 0;
     for (int base = wq.next(); base >= 0; base = wq.next()) {
         const int i = base + __lane_id();
@@ -405,7 +408,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_occlude(SceneView S0, WaveArg
     __shared__ int lds[kLdsStack * kBlockThreads];
     DevStack st = make_stack(lds, A.spill);
     const int n = (int)ctr->n_shadow.v;
-    WaveQueue wq(ctr->sched[2], n);
+    WaveQueue wq(ctr->sched[2], n, A.grab_lg);
     for (int base = wq.next(); base >= 0; base = wq.next()) {
         const int j = base + __lane_id();
         if (j >= n) continue;
@@ -471,14 +474,16 @@ template <bool B>
 __global__ void k_resolve(WaveArgs A, const float4 *samples) {
     const WaveReq &R = req_at<B>(A, blockIdx.y);
     // pixels of the pass: block rows row0 .. row0 + n_samples / (bw 64 spp)
-    const int y0 = R.row0 * 8;
-    const int y1 = R.bw > 0 ? min(R.H, y0 + 8 * (R.n_samples / (R.bw * 64 * R.spp))) : y0;
+    const int lg = R.tile_lg, th_lg = 6 - lg;
+    const int y0 = R.row0 << th_lg;
+    const int y1 = R.bw > 0 ? min(R.H, y0 + ((R.n_samples / (R.bw * 64 * R.spp)) << th_lg)) : y0;
     const int npix = R.W * (y1 - y0);
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < npix; q += gridDim.x * blockDim.x) {
         const int p = y0 * R.W + q;
         const int px = p % R.W, py = p / R.W;
         if (R.tile_compat && (px >= ((R.W >> 6) << 6) || py >= ((R.H >> 6) << 6))) continue;
-        const int pb = ((py >> 3) - R.row0) * R.bw + (px >> 3), lane = ((py & 7) << 3) | (px & 7);
+        const int pb = ((py >> th_lg) - R.row0) * R.bw + (px >> lg);
+        const int lane = ((py & ((1 << th_lg) - 1)) << lg) | (px & ((1 << lg) - 1));
         const int s0 = R.sample_base + (pb * 64 + lane) * R.spp;
         V3 acc = V3{0, 0, 0};
         for (int s = 0; s < R.spp; ++s) {
@@ -534,10 +539,12 @@ static int blocks_per_cu(K kernel) {
 template <bool B>
 struct Grids {
     int extend, shade, occlude;
-    explicit Grids(int cus) {
+    // pct: percent of the resident blocks (at least one block per CU)
+    explicit Grids(int cus, int pct = 100) {
         static const int oe = blocks_per_cu(k_extend<B, false>), os = blocks_per_cu(k_shade<B>),
                          oo = blocks_per_cu(k_occlude<B>);
-        extend = cus * oe, shade = cus * os, occlude = cus * oo;
+        auto part = [&](int o) { return std::max(cus, cus * o * pct / 100); };
+        extend = part(oe), shade = part(os), occlude = part(oo);
     }
 };
 
@@ -617,7 +624,7 @@ template <bool B>
 static int enqueue_wave(const SceneView &S0, const WaveArgs &a, const WavePlan &pl, hipStream_t stream,
                         Workspace &ws, Arena &A, const int *cancel_host, bool retest) {
     const int cus = A.cus > 0 ? A.cus : 256;
-    const Grids<B> g(cus);
+    const Grids<B> g(cus, A.tune.grid_pct);
     if (!pl.no_sync) {
         if (ws.grow(ws.spill, spill_ints(std::max(g.extend, g.occlude)) * sizeof(int)))
             return fail(RTG_E_NOMEM, "spill");
@@ -711,7 +718,9 @@ int launch_wavefront(const SceneRes &S, const rtg_params &p, int W, int H, const
     WaveArgs a{};
     WaveReq &r = a.r0;
     r.W = W, r.H = H, r.spp = p.spp > 0 ? p.spp : 1, r.max_level = p.max_level, r.tile_compat = p.tile_compat;
-    r.bw = (W + 7) / 8;
+    const int lg = A.tune.tile_lg, tw = 1 << lg, th = 64 >> lg;
+    r.tile_lg = lg;
+    r.bw = (W + tw - 1) / tw;
     r.sample_base = 0;
     r.scene = 0;
     r.dpf = (float)dp;
@@ -723,10 +732,11 @@ int launch_wavefront(const SceneRes &S, const rtg_params &p, int W, int H, const
     r.rgb = o.rgb, r.ids = o.ids, r.tprim = o.tprim;
     a.counts = o.counts;
     a.retests = o.retests;
+    a.grab_lg = A.tune.grab_lg;
     // Frames whose ray trees can branch (refraction: two children per hit)
     // run in passes of whole block rows, at most kPassSamples level-0
     // samples each, so that the queues of all levels stay a few GB.
-    const int rows = (H + 7) / 8, per_row = r.bw * 64 * r.spp;
+    const int rows = (H + th - 1) / th, per_row = r.bw * 64 * r.spp;
     const int rows_per_pass = S.max_fanout >= 2 ? std::max(1, kPassSamples / per_row) : rows;
     for (int row0 = 0; row0 < rows; row0 += rows_per_pass) {
         r.row0 = row0;
@@ -790,8 +800,10 @@ int launch_batch(const BatchItem *items, int n, unsigned long long *counts, unsi
         WaveReq &r = reqs[i];
         r.W = it.W, r.H = it.H, r.spp = it.params.spp > 0 ? it.params.spp : 1;
         r.max_level = it.params.max_level, r.tile_compat = it.params.tile_compat;
-        r.bw = (it.W + 7) / 8;
-        r.n_samples = r.bw * ((it.H + 7) / 8) * 64 * r.spp;
+        const int lg = A.tune.tile_lg;
+        r.tile_lg = lg;
+        r.bw = (it.W + (1 << lg) - 1) >> lg;
+        r.n_samples = (int)frame_samples(it.W, it.H, r.spp, lg);
         r.sample_base = (int)total;
         r.scene = scene_of[i];
         r.seed = it.params.seed;
@@ -850,12 +862,14 @@ int launch_batch(const BatchItem *items, int n, unsigned long long *counts, unsi
     a.scenes = ws.b_scenes.as<SceneView>();
     a.chunk_req = ws.b_chunks.as<int>();
     a.n_samples = (int)padded;
+    a.grab_lg = A.tune.grab_lg;
     a.counts = counts;
     a.retests = retests;
     const uint64_t key = (uint64_t)padded ^ ((uint64_t)n_pad << 32) ^ ((uint64_t)max_level << 45) ^
                          ((uint64_t)fanout << 50) ^ ((uint64_t)n_on << 53) ^ ((uint64_t)any_retest << 60) ^
                          (ws.epoch * 0x9E3779B97F4A7C15ull) ^ ((uint64_t)(uintptr_t)counts * 0xC2B2AE3D27D4EB4Full) ^
-                         ((uint64_t)(uintptr_t)retests * 0x165667B19E3779F9ull);
+                         ((uint64_t)(uintptr_t)retests * 0x165667B19E3779F9ull) ^
+                         (A.tune.key() * 0x27D4EB2F165667C5ull);
     auto enqueue = [&]() -> int {
         k_chunk_req<<<(int)((n_chunks + 255) / 256), 256, 0, stream>>>(a.reqs, n_pad, ws.b_chunks.as<int>(),
                                                                       (int)n_chunks);

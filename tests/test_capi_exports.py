@@ -35,5 +35,19 @@ def test_abi_version_and_error_path():
     assert lib.rtg_render(None, None, None, None, None, None) == _capi.RTG_E_INVALID
 
 
+def test_tuning_values_are_validated_before_any_device_call():
+    lib = _capi.load()
+    ok = _capi.rtg_tuning(tile_w=16, grab=2, grid_pct=75)
+    assert lib.rtg_set_tuning(None) == _capi.RTG_E_INVALID
+    for bad in (dict(tile_w=6, grab=1, grid_pct=100), dict(tile_w=64, grab=1, grid_pct=100),
+                dict(tile_w=8, grab=3, grid_pct=100), dict(tile_w=8, grab=1, grid_pct=24),
+                dict(tile_w=8, grab=1, grid_pct=101)):
+        t = _capi.rtg_tuning(**bad)
+        assert lib.rtg_set_tuning(t) == _capi.RTG_E_INVALID, bad
+    # a valid tuning gets as far as the device check (no rtg_init here)
+    assert lib.rtg_set_tuning(ok) == _capi.RTG_E_STATE
+    assert lib.rtg_autotune(None, 0, 3, None, None, 0, None) == _capi.RTG_E_INVALID
+
+
 def test_package_imports_without_gpu():
     assert hasattr(raytrace_amd, "DeviceScene")
