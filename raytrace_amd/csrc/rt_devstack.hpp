@@ -86,7 +86,7 @@ struct WaveQueue {
         for (;;) {
             int c = 0;
             if (lane == 0) c = (int)atomicAdd(&ctr[x].v, 1u);
-            c = __shfl(c, 0, 64);
+            c = __builtin_amdgcn_readfirstlane(c);  // wave-uniform: an SGPR
             if (c < region_len(x)) {
                 cur = (x * region + c) << (6 + glg);
                 end = min(n, cur + (64 << glg));
