@@ -192,7 +192,8 @@ def bench_stress(args) -> dict:
     tx, ty = (W + 63) // 64, (H + 63) // 64
     rng = np.random.default_rng(5)
     tiles = np.sort(rng.choice(tx * ty, size=(tx * ty) // 64, replace=False)).astype(np.int32)
-    threads = max(1, min(64, os.cpu_count() or 1))
+    # host cores this process may use, at most the 16 of a one-GPU share
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
     log(f"FP64 oracle on {len(tiles)} tiles")
     o64 = osc.render(p, po.ORACLE_FP64, threads=threads, tiles=tiles)
     m = np.zeros((H, W), bool)
@@ -238,7 +239,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--engine", default="auto", choices=["mega", "wave", "auto"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-tiles", type=int, default=2, help="1/N of the 64x64 tiles for the CPU baseline sample")
+    ap.add_argument("--cpu-tiles", type=int, default=16, help="1/N of the 64x64 tiles for the CPU baseline sample")
+    ap.add_argument("--cpu-runs", type=int, default=5, help="CPU baseline: median of this many runs")
     ap.add_argument("--workload", type=int, default=2, choices=[2, 3, 4, 5],
                     help="2: the 1080p frame (default line); 3: 256 batched requests; 4: serving sweep; "
                          "5: 4K stress frame")
@@ -346,11 +348,12 @@ def main() -> int:
     osc = po.OracleScene(hs.desc_ptr)
     tx, ty = (W + 63) // 64, (H + 63) // 64
     rng = np.random.default_rng(11)
-    threads = max(1, min(64, os.cpu_count() or 1))
+    # host cores this process may use, at most the 16 of a one-GPU share
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
     traffic_tiles = np.sort(rng.choice(tx * ty, size=(tx * ty) // 16, replace=False)).astype(np.int32)
     tm = traffic_model(osc.traffic(p, threads=threads, tiles=traffic_tiles))
-    # CPU baseline sample: a seeded 1/N of the frame's tiles (about 10 s of
-    # oracle work on 16 host threads at N = 2)
+    # CPU baseline sample: a seeded 1/N of the frame's tiles (N = 16: about
+    # 1.3 s per run of the reference-algorithm port on 16 host threads)
     tiles = np.sort(np.random.default_rng(12).choice(tx * ty, size=max(1, (tx * ty) // args.cpu_tiles),
                                                      replace=False)).astype(np.int32)
     if dom == "k_occlude":
@@ -375,11 +378,27 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu_baseline:
-        out = osc.render(p, po.ORACLE_REF32, threads=threads, tiles=tiles, want=("rgb",))
-        st = out["stats"]
-        cpu = {"value": round(st.rays() / st.seconds / 1e6, 3), "unit": "Mrays/s", "cores": threads,
-               "kind": "port", "sample": f"{len(tiles)} of {tx * ty} 64x64 tiles of the 1080p frame "
-                                         f"({st.pixels} pixels, {st.rays()} rays, {st.seconds:.1f} s)"}
+        def timed(mode, nthreads, runs):
+            """Median Mrays/s over `runs` wall-clock renders of the tile sample."""
+            rates = []
+            for _ in range(runs):
+                st = osc.render(p, mode, threads=nthreads, tiles=tiles, want=("rgb",))["stats"]
+                rates.append(st.rays() / st.seconds / 1e6)
+            return float(np.median(rates)), [round(r, 4) for r in rates], st
+
+        runs = args.cpu_runs
+        v_all, r_all, st = timed(po.ORACLE_REF32, threads, runs)
+        v_8, r_8, _ = timed(po.ORACLE_REF32, 8, runs)
+        v_o, r_o, _ = timed(po.ORACLE_FP64, threads, runs)
+        cpu = {"value": round(v_all, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+               "sample": f"seeded {len(tiles)} of {tx * ty} 64x64 tiles of the 1080p frame ({st.pixels} pixels, "
+                         f"{st.rays()} reference rays per run), median of {runs} wall-clock runs; the rate "
+                         "extrapolates to the full frame",
+               "runs": r_all,
+               "tnum_8": {"value": round(v_8, 3), "cores": 8, "runs": r_8,
+                          "what": "the same port at the reference's default of 8 worker threads"},
+               "oracle": {"value": round(v_o, 3), "cores": threads, "runs": r_o,
+                          "what": "the FP64 BVH oracle (not the reference algorithm), same sample"}}
 
     line = {
         "metric": "Mrays/s (primary+secondary) at 1080p", "value": round(value, 2), "unit": "Mrays/s",

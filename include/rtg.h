@@ -208,13 +208,32 @@ int rtg_abi_version(void);
  * the device.  The scene stays resident until rtg_scene_destroy. */
 int rtg_scene_create(const rtg_scene_desc *desc, rtg_scene_t *out);
 /* Replaces the effect of Scene mutations (Scene.cpp:157-301) on the render:
- * re-upload `desc` into an existing handle (rebuilds the BVH). */
+ * re-upload `desc` into an existing handle.  The BVH work follows what
+ * changed:
+ *   - materials, lights, textures, environment light or camera only
+ *     (ChgMtl, light edits): the shading tables are re-uploaded, the BVH stays;
+ *   - object positions or radii with the same primitive set (MovePos, a model
+ *     or sphere moved): the BVH is refitted over its existing topology;
+ *   - anything else (Switch, AddX, Delete, other triangles): rebuilt. */
 int rtg_scene_update(rtg_scene_t scene, const rtg_scene_desc *desc);
+/* rtg_scene_update that always rebuilds the BVH (a fresh tree after many
+ * refits have moved objects far from where the tree was built). */
+int rtg_scene_rebuild(rtg_scene_t scene, const rtg_scene_desc *desc);
 int rtg_scene_destroy(rtg_scene_t scene);
-/* Device bytes held by the scene (geometry + BVH). */
+/* Device bytes held by the scene (geometry + BVH + the topology kept for refits). */
 int rtg_scene_bytes(rtg_scene_t scene, uint64_t *bytes);
-/* Device time (ms) of the scene's last BVH build. */
+/* Device time (ms) of the scene's last BVH build or refit. */
 int rtg_scene_build_ms(rtg_scene_t scene, double *ms);
+/* Size of a resident scene and what its updates did. */
+typedef struct rtg_scene_info {
+    uint64_t bytes;
+    double build_ms;
+    int32_t n_nodes, n_prims;
+    uint64_t rebuilds;        /* updates that rebuilt the BVH */
+    uint64_t refits;          /* updates that refitted it */
+    uint64_t shading_updates; /* updates that re-uploaded shading tables only */
+} rtg_scene_info;
+int rtg_scene_stats(rtg_scene_t scene, rtg_scene_info *out);
 
 /* ---- render ----------------------------------------------------------- */
 

@@ -156,6 +156,16 @@ class DeviceScene:
     def update(self, host: HostScene) -> None:
         check(self._lib.rtg_scene_update(self.h, C.c_void_p(host.desc_ptr)), "rtg_scene_update")
 
+    def rebuild(self, host: HostScene) -> None:
+        """rtg_scene_rebuild: update with a fresh BVH (no refit)."""
+        check(self._lib.rtg_scene_rebuild(self.h, C.c_void_p(host.desc_ptr)), "rtg_scene_rebuild")
+
+    def stats(self) -> dict:
+        """rtg_scene_stats: size and what the updates did."""
+        info = _capi.rtg_scene_info()
+        check(self._lib.rtg_scene_stats(self.h, C.byref(info)), "rtg_scene_stats")
+        return {k: getattr(info, k) for k, _ in info._fields_}
+
     @property
     def build_ms(self) -> float:
         v = C.c_double()

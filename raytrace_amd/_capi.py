@@ -163,6 +163,11 @@ class rtg_request(C.Structure):
     ]
 
 
+class rtg_scene_info(C.Structure):
+    _fields_ = [("bytes", C.c_uint64), ("build_ms", C.c_double), ("n_nodes", C.c_int32), ("n_prims", C.c_int32),
+                ("rebuilds", C.c_uint64), ("refits", C.c_uint64), ("shading_updates", C.c_uint64)]
+
+
 class rtg_tuning(C.Structure):
     _fields_ = [("tile_w", C.c_int32), ("grab", C.c_int32), ("grid_pct", C.c_int32), ("reserved", C.c_int32)]
 
@@ -184,6 +189,8 @@ SIGNATURES = {
     "rtg_abi_version": (_I, []),
     "rtg_scene_create": (_I, [_P, C.POINTER(_P)]),
     "rtg_scene_update": (_I, [_P, _P]),
+    "rtg_scene_rebuild": (_I, [_P, _P]),
+    "rtg_scene_stats": (_I, [_P, C.POINTER(rtg_scene_info)]),
     "rtg_scene_destroy": (_I, [_P]),
     "rtg_scene_bytes": (_I, [_P, C.POINTER(C.c_uint64)]),
     "rtg_scene_build_ms": (_I, [_P, C.POINTER(C.c_double)]),

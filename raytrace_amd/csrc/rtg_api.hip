@@ -43,6 +43,19 @@ uint64_t geometry_hash(const rtg_scene_desc &d) {
     return h;
 }
 
+// Primitive set and order of a scene (what a BVH topology depends on): each
+// object's type, visibility and part range, and each part's triangle count.
+uint64_t topology_hash(const rtg_scene_desc &d) {
+    uint64_t h = 0x9E3779B97F4A7C15ull ^ (uint64_t)d.n_objects;
+    for (int i = 0; i < d.n_objects; ++i) {
+        const rtg_object &o = d.objects[i];
+        const int32_t v[4] = {o.type, o.show, o.part_begin, o.part_count};
+        h = hash_bytes(h, v, sizeof(v));
+    }
+    for (int i = 0; i < d.n_parts; ++i) h = hash_bytes(h, &d.parts[i].tri_count, sizeof(int32_t));
+    return h;
+}
+
 int validate_desc(const rtg_scene_desc *d) {
     if (d == nullptr) return fail(RTG_E_INVALID, "scene descriptor is NULL");
     if (d->n_objects < 0 || d->n_parts < 0 || d->n_materials < 0 || d->n_lights < 0 || d->n_textures < 0)
@@ -295,11 +308,15 @@ int rtg_scene_create(const rtg_scene_desc *desc, rtg_scene_t *out) {
     }
     if (st != RTG_OK) return st;
     s->res.geom_hash = geometry_hash(*desc);
+    s->res.topo_hash = topology_hash(*desc);
     *out = s.release();
     return RTG_OK;
 }
 
-int rtg_scene_update(rtg_scene_t scene, const rtg_scene_desc *desc) {
+// Update a resident scene: shading tables only when the geometry is the same;
+// a BVH refit when only positions or shapes moved (same primitive set and
+// order) and `allow_refit`; a full rebuild otherwise.
+static int scene_update(rtg_scene_t scene, const rtg_scene_desc *desc, bool allow_refit) {
     if (scene == nullptr) return fail(RTG_E_INVALID, "scene is NULL");
     int st = validate_desc(desc);
     if (st != RTG_OK) return st;
@@ -308,31 +325,27 @@ int rtg_scene_update(rtg_scene_t scene, const rtg_scene_desc *desc) {
     std::lock_guard<std::mutex> l(A.lock);
     // Same geometry (objects, parts, triangles): only materials, lights,
     // textures and the camera can have changed, so the BVH stays.
-    const uint64_t gh = geometry_hash(*desc);
-    SceneRes &R0 = scene->res;
-    if (R0.geom_hash != 0 && gh == R0.geom_hash) {
-        ++R0.shading_updates;
-        return scene_update_shading(R0, *desc, A.stream);
+    const uint64_t gh = geometry_hash(*desc), th = topology_hash(*desc);
+    SceneRes &R = scene->res;
+    if (allow_refit && R.geom_hash != 0 && gh == R.geom_hash) {
+        ++R.shading_updates;
+        return scene_update_shading(R, *desc, A.stream);
     }
+    const bool refit = allow_refit && R.topo_n > 0 && th == R.topo_hash;
     SceneRes fresh;
     fresh.device = A.device;
-    st = scene_build(fresh, *desc, A.stream);
+    st = scene_build(fresh, *desc, A.stream, refit ? &R : nullptr);
     if (st != RTG_OK) return st;
-    SceneRes &R = scene->res;
-    std::swap(R.nodes.p, fresh.nodes.p), std::swap(R.nodes.bytes, fresh.nodes.bytes);
-    std::swap(R.prims.p, fresh.prims.p), std::swap(R.prims.bytes, fresh.prims.bytes);
-    std::swap(R.objs.p, fresh.objs.p), std::swap(R.objs.bytes, fresh.objs.bytes);
-    std::swap(R.brute.p, fresh.brute.p), std::swap(R.brute.bytes, fresh.brute.bytes);
-    std::swap(R.parts.p, fresh.parts.p), std::swap(R.parts.bytes, fresh.parts.bytes);
-    std::swap(R.tri_nrm.p, fresh.tri_nrm.p), std::swap(R.tri_nrm.bytes, fresh.tri_nrm.bytes);
-    std::swap(R.tri_uv.p, fresh.tri_uv.p), std::swap(R.tri_uv.bytes, fresh.tri_uv.bytes);
-    std::swap(R.tri_pos.p, fresh.tri_pos.p), std::swap(R.tri_pos.bytes, fresh.tri_pos.bytes);
-    std::swap(R.tri_part.p, fresh.tri_part.p), std::swap(R.tri_part.bytes, fresh.tri_part.bytes);
-    std::swap(R.tri_bins.p, fresh.tri_bins.p), std::swap(R.tri_bins.bytes, fresh.tri_bins.bytes);
-    std::swap(R.mtls.p, fresh.mtls.p), std::swap(R.mtls.bytes, fresh.mtls.bytes);
-    std::swap(R.lights.p, fresh.lights.p), std::swap(R.lights.bytes, fresh.lights.bytes);
-    std::swap(R.texs.p, fresh.texs.p), std::swap(R.texs.bytes, fresh.texs.bytes);
-    std::swap(R.texels.p, fresh.texels.p), std::swap(R.texels.bytes, fresh.texels.bytes);
+    auto take = [](DevBuf &a, DevBuf &b) { std::swap(a.p, b.p), std::swap(a.bytes, b.bytes); };
+    take(R.nodes, fresh.nodes), take(R.prims, fresh.prims), take(R.objs, fresh.objs), take(R.brute, fresh.brute);
+    take(R.parts, fresh.parts), take(R.tri_nrm, fresh.tri_nrm), take(R.tri_uv, fresh.tri_uv);
+    take(R.tri_pos, fresh.tri_pos), take(R.tri_part, fresh.tri_part), take(R.tri_bins, fresh.tri_bins);
+    take(R.mtls, fresh.mtls), take(R.lights, fresh.lights), take(R.texs, fresh.texs), take(R.texels, fresh.texels);
+    take(R.t_keys, fresh.t_keys), take(R.t_left, fresh.t_left), take(R.t_right, fresh.t_right);
+    take(R.t_first, fresh.t_first), take(R.t_last, fresh.t_last), take(R.t_pint, fresh.t_pint);
+    take(R.t_pleaf, fresh.t_pleaf), take(R.t_newidx, fresh.t_newidx);
+    R.topo_n = fresh.topo_n, R.topo_emit = fresh.topo_emit;
+    R.topo_hash = th;
     R.view = fresh.view;
     R.camera = fresh.camera;
     R.build_ms = fresh.build_ms;
@@ -341,7 +354,27 @@ int rtg_scene_update(rtg_scene_t scene, const rtg_scene_desc *desc) {
     R.lights_on = fresh.lights_on;
     R.max_fanout = fresh.max_fanout;
     R.geom_hash = gh;
-    ++R.rebuilds;
+    if (refit && fresh.topo_n > 0)
+        ++R.refits;
+    else
+        ++R.rebuilds;
+    return RTG_OK;
+}
+
+int rtg_scene_update(rtg_scene_t scene, const rtg_scene_desc *desc) { return scene_update(scene, desc, true); }
+
+int rtg_scene_rebuild(rtg_scene_t scene, const rtg_scene_desc *desc) { return scene_update(scene, desc, false); }
+
+int rtg_scene_stats(rtg_scene_t scene, rtg_scene_info *out) {
+    if (scene == nullptr || out == nullptr) return fail(RTG_E_INVALID, "NULL argument");
+    const SceneRes &R = scene->res;
+    out->bytes = R.bytes;
+    out->build_ms = R.build_ms;
+    out->n_nodes = R.n_nodes;
+    out->n_prims = R.view.n_prims;
+    out->rebuilds = R.rebuilds;
+    out->refits = R.refits;
+    out->shading_updates = R.shading_updates;
     return RTG_OK;
 }
 

@@ -175,6 +175,14 @@ int upload(DevBuf &b, const std::vector<T> &v, hipStream_t s) {
 
 }  // namespace
 
+// Device bytes a resident scene holds: render data plus the kept topology.
+static size_t scene_bytes(const SceneRes &S) {
+    return S.nodes.bytes + S.prims.bytes + S.objs.bytes + S.brute.bytes + S.parts.bytes + S.tri_nrm.bytes +
+           S.tri_uv.bytes + S.tri_pos.bytes + S.tri_part.bytes + S.tri_bins.bytes + S.mtls.bytes + S.lights.bytes +
+           S.texs.bytes + S.texels.bytes + S.t_keys.bytes + S.t_left.bytes + S.t_right.bytes + S.t_first.bytes +
+           S.t_last.bytes + S.t_pint.bytes + S.t_pleaf.bytes + S.t_newidx.bytes;
+}
+
 // Lights switched on and the most child rays one hit can spawn (reflection
 // and/or refraction) over the materials of shown objects: the latter bounds
 // the wavefront queues per level.
@@ -199,7 +207,7 @@ static void shading_state(SceneRes &S, const rtg_scene_desc &d) {
 
 }
 
-int scene_build(SceneRes &S, const rtg_scene_desc &d, hipStream_t stream) {
+int scene_build(SceneRes &S, const rtg_scene_desc &d, hipStream_t stream, SceneRes *donor) {
     ScenePrep prep;
     const std::string err = prepare_scene(d, prep);
     if (!err.empty()) return fail(RTG_E_INVALID, err);
@@ -221,8 +229,20 @@ int scene_build(SceneRes &S, const rtg_scene_desc &d, hipStream_t stream) {
         (st = upload(S.texels, texels, stream)) || (st = upload(S.tri_part, tri_part, stream)))
         return st;
     const int n = (int)(n_tris + (int64_t)spheres.size());
-    DevBuf raw, mid, sph, prims_u, boxes, keys, keys2, left, right, first, last, pint, pleaf, flags, ibox, newidx,
-        partial, cb, sort_tmp;
+    // refit: the donor's tree has the same primitive count (and the caller
+    // checked the same primitive order); its topology moves into S
+    const bool refit = donor != nullptr && donor->topo_n == n && n > kLeafMax;
+    if (refit) {
+        DevBuf *mine[] = {&S.t_keys, &S.t_left, &S.t_right, &S.t_first, &S.t_last, &S.t_pint, &S.t_pleaf, &S.t_newidx};
+        DevBuf *theirs[] = {&donor->t_keys, &donor->t_left,  &donor->t_right, &donor->t_first,
+                            &donor->t_last, &donor->t_pint, &donor->t_pleaf, &donor->t_newidx};
+        for (int k = 0; k < 8; ++k) std::swap(mine[k]->p, theirs[k]->p), std::swap(mine[k]->bytes, theirs[k]->bytes);
+        S.topo_n = donor->topo_n, S.topo_emit = donor->topo_emit;
+        donor->topo_n = 0, donor->topo_emit = 0;
+    }
+    DevBuf raw, mid, sph, prims_u, boxes, keys, flags, ibox, partial, cb, sort_tmp;
+    DevBuf &keys2 = S.t_keys, &left = S.t_left, &right = S.t_right, &first = S.t_first, &last = S.t_last,
+           &pint = S.t_pint, &pleaf = S.t_pleaf, &newidx = S.t_newidx;
     if ((st = upload(mid, part_mid, stream)) || (st = upload(sph, spheres, stream))) return st;
     if (ensure(raw, std::max<size_t>(n_tris, 1) * sizeof(rtg_triangle))) return RTG_E_NOMEM;
     {
@@ -257,7 +277,24 @@ int scene_build(SceneRes &S, const rtg_scene_desc &d, hipStream_t stream) {
                                                                      S.objs.as<DevObject>(), prims_u.as<float4>(),
                                                                      boxes.as<Box6>());
     int root = 0, n_emit = 0;
-    if (n > kLeafMax) {
+    if (refit) {
+        // same topology: leaf boxes from the new primitives, bottom-up union,
+        // nodes re-emitted with the same links, primitives into leaf order
+        if (ensure(flags, n * sizeof(int)) || ensure(ibox, n * sizeof(Box6)))
+            return fail(RTG_E_NOMEM, "device allocation failed (bvh refit)");
+        const uint64_t *sk = keys2.as<uint64_t>();
+        k_fill_i32<<<nblk(n), kBlock, 0, stream>>>(flags.as<int>(), n, 0);
+        k_refit<<<nblk(n), kBlock, 0, stream>>>(sk, n, boxes.as<Box6>(), left.as<int>(), right.as<int>(), pint.as<int>(),
+                                                pleaf.as<int>(), flags.as<int>(), ibox.as<Box6>());
+        n_emit = S.topo_emit;
+        if (ensure(S.nodes, std::max(n_emit, 1) * sizeof(BvhNode))) return fail(RTG_E_NOMEM, "bvh nodes");
+        k_emit<<<nblk(n - 1), kBlock, 0, stream>>>(sk, n - 1, boxes.as<Box6>(), left.as<int>(), right.as<int>(),
+                                                   first.as<int>(), last.as<int>(), ibox.as<Box6>(), newidx.as<int>(),
+                                                   S.nodes.as<BvhNode>());
+        if (ensure(S.prims, (size_t)n * 3 * sizeof(float4))) return fail(RTG_E_NOMEM, "prims");
+        k_reorder<<<nblk(n), kBlock, 0, stream>>>(sk, n, prims_u.as<float4>(), S.prims.as<float4>());
+        root = 0;
+    } else if (n > kLeafMax) {
         const int nb = 512;
         if (ensure(partial, nb * 6 * sizeof(float)) || ensure(cb, 6 * sizeof(float)) ||
             ensure(keys, n * sizeof(uint64_t)) || ensure(keys2, n * sizeof(uint64_t)) ||
@@ -302,6 +339,7 @@ int scene_build(SceneRes &S, const rtg_scene_desc &d, hipStream_t stream) {
         if (ensure(S.prims, (size_t)n * 3 * sizeof(float4))) return fail(RTG_E_NOMEM, "prims");
         k_reorder<<<nblk(n), kBlock, 0, stream>>>(sk, n, prims_u.as<float4>(), S.prims.as<float4>());
         root = 0;
+        S.topo_n = n, S.topo_emit = n_emit;
     } else {
         // few primitives: one leaf, prims in input order
         if (ensure(S.prims, std::max(n, 1) * 3 * sizeof(float4)) || ensure(S.nodes, sizeof(BvhNode)))
@@ -347,9 +385,7 @@ int scene_build(SceneRes &S, const rtg_scene_desc &d, hipStream_t stream) {
     v.env = prep.env;
     v.cam = prep.cam;
     S.camera = d.camera;
-    S.bytes = S.nodes.bytes + S.prims.bytes + S.objs.bytes + S.brute.bytes + S.parts.bytes + S.tri_nrm.bytes +
-              S.tri_uv.bytes + S.tri_pos.bytes + S.tri_part.bytes + S.tri_bins.bytes + S.mtls.bytes + S.lights.bytes + S.texs.bytes +
-              S.texels.bytes;
+    S.bytes = scene_bytes(S);
     return RTG_OK;
 }
 
@@ -379,9 +415,7 @@ int scene_update_shading(SceneRes &S, const rtg_scene_desc &d, hipStream_t strea
     v.env = prep.env;
     v.cam = prep.cam;
     S.camera = d.camera;
-    S.bytes = S.nodes.bytes + S.prims.bytes + S.objs.bytes + S.brute.bytes + S.parts.bytes + S.tri_nrm.bytes +
-              S.tri_uv.bytes + S.tri_pos.bytes + S.tri_part.bytes + S.tri_bins.bytes + S.mtls.bytes + S.lights.bytes +
-              S.texs.bytes + S.texels.bytes;
+    S.bytes = scene_bytes(S);
     return RTG_OK;
 }
 

@@ -49,7 +49,15 @@ struct SceneRes {
     uint32_t lights_on = 0;  // bit i: light i is switched on
     int max_fanout = 2;      // most child rays per hit over the shown objects' materials
     uint64_t rebuilds = 0;         // rtg_scene_update calls that rebuilt the BVH
+    uint64_t refits = 0;           // rtg_scene_update calls that refitted the BVH
     uint64_t shading_updates = 0;  // rtg_scene_update calls that only re-uploaded shading tables
+    // Tree topology of the last full build, kept for refits: sorted keys
+    // (primitive id in the low 32 bits), radix-tree children, leaf ranges,
+    // parents and emitted-node indices of the Karras tree over topo_n
+    // primitives; topo_hash identifies the primitive set and order.
+    DevBuf t_keys, t_left, t_right, t_first, t_last, t_pint, t_pleaf, t_newidx;
+    int topo_n = 0, topo_emit = 0;
+    uint64_t topo_hash = 0;
 };
 
 // Device memory of one in-flight render (or batch): traversal spill stacks,
@@ -162,7 +170,11 @@ int persistent_blocks(const Arena &A);
 Arena *arena();
 int ensure(DevBuf &b, size_t bytes);
 
-int scene_build(SceneRes &S, const rtg_scene_desc &d, hipStream_t stream);
+// Build S from d.  With `donor` holding the topology of a tree over the same
+// primitive set (same count and order), that topology moves into S and the
+// tree is refitted: primitives and boxes are recomputed, the sort and the
+// radix tree are skipped.
+int scene_build(SceneRes &S, const rtg_scene_desc &d, hipStream_t stream, SceneRes *donor = nullptr);
 // Update of a scene whose geometry is unchanged: shading tables only.
 int scene_update_shading(SceneRes &S, const rtg_scene_desc &d, hipStream_t stream);
 // Frame size of a render (params or the scene camera's) and parameter checks.

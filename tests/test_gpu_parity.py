@@ -313,6 +313,48 @@ def test_scene_update_paths(rt_gpu):
     ds.update(hs)
     assert ds.build_ms == b0
     same_as_fresh()
-    hs.move_pos(2, 2, (0.5, 0.25, -0.5))  # MovePos(MY_MODEL_OBJECT, 2): geometry changed
+    hs.move_pos(2, 2, (0.5, 0.25, -0.5))  # MovePos(MY_MODEL_OBJECT, 2): same primitives, moved
     ds.update(hs)
+    st = ds.stats()
+    assert st["refits"] == 1 and st["rebuilds"] == 0 and st["shading_updates"] == 2
     same_as_fresh()
+
+
+def test_scene_refit_on_move(rt_gpu):
+    """MovePos (Scene.cpp:190-230) keeps the primitive set, so rtg_scene_update
+    refits the BVH over its kept topology instead of rebuilding it; every
+    refitted scene renders exactly as a freshly built one (and as the
+    reference algorithm).  Switch changes the primitive set: rebuilt.
+    rtg_scene_rebuild always rebuilds."""
+    rt = rt_gpu
+    log("case: workload 2 (480x270): four MovePos refits, a Switch rebuild, a forced rebuild")
+    hs = rt.HostScene.workload(2, width=480, height=270)
+    ds = rt.DeviceScene(hs)
+    p = rt.params(480, 270, max_level=4)
+
+    def same_as_fresh(what):
+        g, f = ds.render(p), rt.DeviceScene(hs).render(p)
+        assert (g["rgb"] == f["rgb"]).all() and (g["ids"] == f["ids"]).all(), what
+        assert g["stats"].rays_total == f["stats"].rays_total, what
+        return g
+
+    moves = [(0, (0.3, 0.0, 0.0)), (3, (0.0, 1.5, 0.0)), (0, (-1.0, 0.5, 0.2)), (10, (2.0, -0.5, 1.0))]
+    for k, (obj, v) in enumerate(moves):
+        hs.move_pos(2, obj, v)
+        ds.update(hs)
+        st = ds.stats()
+        log(f"  move {k}: object {obj} by {v}: {st}")
+        assert st["refits"] == k + 1 and st["rebuilds"] == 0
+        g = same_as_fresh(f"move {k}")
+    tiles = np.arange(0, 40, 3, dtype=np.int32)
+    m = tile_mask(480, 270, tiles)
+    _check(compare_frames(g, _oracle(hs, p, tiles=tiles), region=m))
+    hs.switch(2, 3, False)
+    ds.update(hs)
+    assert ds.stats()["rebuilds"] == 1
+    same_as_fresh("switch")
+    hs.move_pos(2, 0, (0.1, 0.1, 0.1))
+    ds.rebuild(hs)
+    st = ds.stats()
+    assert st["rebuilds"] == 2 and st["refits"] == 4
+    same_as_fresh("rebuild")
