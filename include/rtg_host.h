@@ -94,6 +94,13 @@ int rtg_host_resize(rtg_host_scene_t scene, int width, int height);
 int rtg_host_raytrace(rtg_host_scene_t scene, int type, int max_level, int frames, uint8_t *out,
                       double *use_time, uint64_t *uploads);
 
+/* Write a frame as an 8-bit RGB PNG: what the reference's viewer shows of
+ * RayTracer::output (main.cpp:207-208, an RGB texture whose row 0 is the
+ * bottom).  `stride` bytes per source row (0: 3 * width); bottom_up != 0 flips
+ * the rows so that the file's first row is the frame's top.  Returns 0, -1
+ * for bad arguments, -2 when the file cannot be written. */
+int rtg_host_write_png(const char *path, const uint8_t *rgb, int width, int height, int stride, int bottom_up);
+
 #ifdef __cplusplus
 }
 #endif

@@ -273,6 +273,19 @@ def autotune(items, reps: int = 3):
                                 for i in range(min(nt.value, cap))]
 
 
+def write_png(path: str, rgb: np.ndarray, bottom_up: bool = True) -> None:
+    """rtg_host_write_png: an H x W x 3 uint8 frame to an RGB PNG; with
+    bottom_up (the reference's framebuffer order) row 0 becomes the image's
+    bottom row."""
+    a = np.ascontiguousarray(rgb, dtype=np.uint8)
+    if a.ndim != 3 or a.shape[2] != 3:
+        raise ValueError("rgb must be H x W x 3")
+    rc = _capi.load().rtg_host_write_png(path.encode(), a.ctypes.data_as(C.c_void_p), a.shape[1], a.shape[0], 0,
+                                          int(bottom_up))
+    if rc != 0:
+        raise OSError(f"rtg_host_write_png({path!r}) failed ({rc})")
+
+
 def gen_requests(n: int, rate: float = 0.0, seed: int = 3):
     """Request stream of workloads 3/4: [(scene index, width, height, arrival_s)]."""
     lib = _capi.load()

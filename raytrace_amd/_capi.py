@@ -212,6 +212,7 @@ SIGNATURES = {
     "rtg_host_scene_config": (_I, [_I, C.POINTER(rtg_gen_opts), C.POINTER(_P), C.POINTER(rtg_gen_info)]),
     "rtg_host_scene_new": (_I, [C.POINTER(_P)]),
     "rtg_host_gen_requests": (_I, [C.c_uint64, _I, C.c_double, C.POINTER(rtg_gen_request)]),
+    "rtg_host_write_png": (_I, [C.c_char_p, _P, _I, _I, _I, _I]),
     "rtg_host_scene_destroy": (_I, [_P]),
     "rtg_host_scene_desc": (_P, [_P]),
     "rtg_host_add_sphere": (_I, [_P, _F]),

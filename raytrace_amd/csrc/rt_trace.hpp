@@ -410,12 +410,12 @@ RTG_HD Hit trace_closest(const SceneView &S, const TraceRay &q, Stack &st, int *
 // the running closest hit before the source object's turn in the scan, i.e.
 // the closest hit among the objects scanned before it, or the source identity
 // when none of them was hit.  Returns obj == -2 when it is the winner.
-RTG_HD inline bool is_self_rehit(const TraceRay &q, const Hit &h) {
+RTG_HD bool is_self_rehit(const TraceRay &q, const Hit &h) {
     return q.src.obj >= 0 && h.obj == q.src.obj &&
            (h.kind == HIT_SPHERE || (h.kind == HIT_BALL && h.prim == q.src.prim));
 }
 // The identity from `b`, the closest hit among the objects before the source.
-RTG_HD inline SrcId child_src_of(const SceneView &S, const TraceRay &q, const Hit &b) {
+RTG_HD SrcId child_src_of(const SceneView &S, const TraceRay &q, const Hit &b) {
     if (b.kind == HIT_NONE) return q.src;
     SrcId id{b.obj, b.kind == HIT_TRI || b.kind == HIT_BALL ? b.prim : -1, -1};
     if (b.kind == HIT_TRI) id.bin = record_bin(S, q.r, q.src, b.prim);

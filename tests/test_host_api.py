@@ -155,3 +155,33 @@ def test_scene_descriptor_is_deterministic():
     for k in range(a.n_objects):
         assert bytes(a.objects[k]) == bytes(b.objects[k])
     assert bytes(a.lights[0]) == bytes(b.lights[0])
+
+
+def test_png_writer_round_trip(tmp_path):
+    """rtg_host_write_png: the frame decodes back (standard zlib, chunk CRCs)
+    to the same pixels, with the reference's bottom-up rows flipped."""
+    import struct
+    import zlib
+    rng = np.random.default_rng(3)
+    for (h, w) in ((1, 1), (7, 5), (200, 300)):  # 300 x 200: three stored deflate blocks
+        img = rng.integers(0, 256, (h, w, 3), dtype=np.uint8)
+        for bottom_up in (False, True):
+            path = str(tmp_path / f"f_{h}_{w}_{int(bottom_up)}.png")
+            rt.write_png(path, img, bottom_up=bottom_up)
+            data = open(path, "rb").read()
+            assert data[:8] == b"\x89PNG\r\n\x1a\n"
+            pos, chunks = 8, []
+            while pos < len(data):
+                (n,) = struct.unpack(">I", data[pos:pos + 4])
+                typ, body = data[pos + 4:pos + 8], data[pos + 8:pos + 8 + n]
+                (crc,) = struct.unpack(">I", data[pos + 8 + n:pos + 12 + n])
+                assert zlib.crc32(typ + body) == crc, typ
+                chunks.append((typ, body))
+                pos += 12 + n
+            assert [c[0] for c in chunks] == [b"IHDR", b"IDAT", b"IEND"]
+            assert struct.unpack(">IIBBBBB", chunks[0][1]) == (w, h, 8, 2, 0, 0, 0)
+            raw = zlib.decompress(chunks[1][1])
+            rows = np.frombuffer(raw, np.uint8).reshape(h, 1 + 3 * w)
+            assert (rows[:, 0] == 0).all()
+            got = rows[:, 1:].reshape(h, w, 3)
+            assert (got == (img[::-1] if bottom_up else img)).all()
