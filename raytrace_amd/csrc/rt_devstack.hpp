@@ -31,6 +31,7 @@ struct DevStack {
     __device__ bool empty() const { return sp == 0; }
     __device__ void on_node() {}
     __device__ void on_prim() {}
+    __device__ bool all_lanes(bool v) const { return __all(v); }  // over the wave's active lanes
 };
 
 __device__ inline DevStack make_stack(int *lds_base, int *spill) {

@@ -69,48 +69,91 @@ struct ArrayStack {
     RTG_HD bool empty() const { return sp == 0; }
     RTG_HD void on_node() {}
     RTG_HD void on_prim() {}
+    RTG_HD bool all_lanes(bool v) const { return v; }  // one lane
 };
+
+constexpr int kWalkDone = (int)0x80000000;  // no node left: below every leaf code
 
 // Visit the primitives of every leaf the segment [0, tmax] reaches, nearer
 // child first.  visit(prim_index, tmax&) returns true to stop.  The stack is
 // emptied on entry: an any-hit walk that stops early leaves entries behind,
 // and the stack only holds one entry per level of the current path (a tree
 // of depth < kStackMax) when every walk starts from an empty stack.
-template <class Stack, class Visit>
+//
+// kSpec: speculative traversal (Aila and Laine 2009).  A lane that reaches a
+// leaf postpones it and keeps walking nodes until it reaches a second leaf;
+// the node loop ends when every lane of the wave holds a leaf
+// (Stack::all_lanes), then the held leaves are tested.  The lanes of a wave
+// then spend fewer steps waiting for one another.  A single lane
+// (all_lanes(v) == v) visits in exactly the order of the plain walk.  Only
+// for walks whose result does not depend on the order leaves are tested in
+// (the any-hit test; the closest hit, whose winner is the least (t, key)).
+template <bool kSpec = false, class Stack, class Visit>
 RTG_HD void bvh_walk(const SceneView &S, const Ray &r, float &tmax, Stack &st, Visit &&visit) {
     st.reset();
     if (S.n_prims == 0) return;
     const V3 inv = safe_inv(r.d);
     int node = S.root;
-    for (;;) {
-        while (!is_leaf(node)) {
-            st.on_node();
-            const BvhNode &n = S.nodes[node];
-            const float4 a = n.xy0, b = n.xy1, z = n.z01;
-            const int4 l = n.link;
-            float t0, t1;
-            const bool h0 = slab(r.o, inv, a.x, a.y, a.z, a.w, z.x, z.y, tmax, t0);
-            const bool h1 = slab(r.o, inv, b.x, b.y, b.z, b.w, z.z, z.w, tmax, t1);
-            if (h0 && h1) {
-                const bool first0 = t0 <= t1;
-                st.push(first0 ? l.y : l.x);
-                node = first0 ? l.x : l.y;
-            } else if (h0) {
-                node = l.x;
-            } else if (h1) {
-                node = l.y;
-            } else {
-                if (st.empty()) return;
-                node = st.pop();
+    auto step = [&](int nd) -> int {  // one internal node: the next node, pushing the far child
+        st.on_node();
+        const BvhNode &n = S.nodes[nd];
+        const float4 a = n.xy0, b = n.xy1, z = n.z01;
+        const int4 l = n.link;
+        float t0, t1;
+        const bool h0 = slab(r.o, inv, a.x, a.y, a.z, a.w, z.x, z.y, tmax, t0);
+        const bool h1 = slab(r.o, inv, b.x, b.y, b.z, b.w, z.z, z.w, tmax, t1);
+        if (h0 && h1) {
+            const bool first0 = t0 <= t1;
+            st.push(first0 ? l.y : l.x);
+            return first0 ? l.x : l.y;
+        }
+        if (h0) return l.x;
+        if (h1) return l.y;
+        return st.empty() ? kWalkDone : st.pop();
+    };
+    if constexpr (!kSpec) {
+        for (;;) {
+            while (node >= 0) {
+                node = step(node);
+                if (node == kWalkDone) return;
             }
+            const int first = leaf_first(node), count = leaf_count(node);
+            for (int i = 0; i < count; ++i) {
+                st.on_prim();
+                if (visit(first + i, tmax)) return;
+            }
+            if (st.empty()) return;
+            node = st.pop();
         }
-        const int first = leaf_first(node), count = leaf_count(node);
-        for (int i = 0; i < count; ++i) {
-            st.on_prim();
-            if (visit(first + i, tmax)) return;
+    } else {
+        int leaf = 0;  // postponed leaf (< 0), or 0
+        for (;;) {
+            while (node >= 0) {
+                node = step(node);
+                if (node < 0 && node != kWalkDone && leaf == 0) {
+                    leaf = node;
+                    node = st.empty() ? kWalkDone : st.pop();
+                }
+                if (st.all_lanes(leaf != 0)) break;
+            }
+            if (leaf == 0 && node < 0 && node != kWalkDone) {
+                leaf = node;
+                node = st.empty() ? kWalkDone : st.pop();
+            }
+            while (leaf < 0) {
+                const int first = leaf_first(leaf), count = leaf_count(leaf);
+                for (int i = 0; i < count; ++i) {
+                    st.on_prim();
+                    if (visit(first + i, tmax)) return;
+                }
+                leaf = 0;
+                if (node < 0 && node != kWalkDone) {
+                    leaf = node;
+                    node = st.empty() ? kWalkDone : st.pop();
+                }
+            }
+            if (node == kWalkDone) return;
         }
-        if (st.empty()) return;
-        node = st.pop();
     }
 }
 
@@ -320,7 +363,7 @@ RTG_HD Hit trace_closest(const SceneView &S, const TraceRay &q, Stack &st, int *
             t2 = t, pi2 = pi;
         }
     };
-    bvh_walk(S, q.r, tmax, st, [&](int pi, float &tm) -> bool {
+    bvh_walk<!kRetest>(S, q.r, tmax, st, [&](int pi, float &tm) -> bool {
         const float4 A = S.prims[3 * pi];
         const int tag = as_int(A.w);
         const V3 p = V3{A.x, A.y, A.z};
@@ -434,7 +477,7 @@ RTG_HD bool trace_any(const SceneView &S, const TraceRay &q, float dis, Stack &s
     if (brute_objects(S, q, true, dis, dummy)) return true;
     bool blocked = false;
     float tmax = dis;
-    bvh_walk(S, q.r, tmax, st, [&](int pi, float &) -> bool {
+    bvh_walk<true>(S, q.r, tmax, st, [&](int pi, float &) -> bool {
         const float4 A = S.prims[3 * pi];
         const int tag = as_int(A.w);
         const V3 p = V3{A.x, A.y, A.z};

@@ -42,6 +42,7 @@ struct TrackStack {
     }
     int pop() { return d[--sp]; }
     bool empty() const { return sp == 0; }
+    bool all_lanes(bool v) const { return v; }
 };
 
 struct HostScene {
