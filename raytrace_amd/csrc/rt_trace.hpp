@@ -81,9 +81,9 @@ constexpr int kWalkDone = (int)0x80000000;  // no node left: below every leaf co
 // of depth < kStackMax) when every walk starts from an empty stack.
 //
 // kSpec: speculative traversal (Aila and Laine 2009).  A lane that reaches a
-// leaf postpones it and keeps walking nodes until it reaches a second leaf;
-// the node loop ends when every lane of the wave holds a leaf
-// (Stack::all_lanes), then the held leaves are tested.  The lanes of a wave
+// leaf postpones it and keeps walking nodes, postponing a second leaf too,
+// until it reaches a third; the node loop ends when every lane of the wave
+// holds a leaf (Stack::all_lanes), then the held leaves are tested.  The lanes of a wave
 // then spend fewer steps waiting for one another.  A single lane
 // (all_lanes(v) == v) visits in exactly the order of the plain walk.  Only
 // for walks whose result does not depend on the order leaves are tested in
@@ -126,12 +126,12 @@ RTG_HD void bvh_walk(const SceneView &S, const Ray &r, float &tmax, Stack &st, V
             node = st.pop();
         }
     } else {
-        int leaf = 0;  // postponed leaf (< 0), or 0
+        int leaf = 0, leaf2 = 0;  // postponed leaves (< 0), or 0
         for (;;) {
             while (node >= 0) {
                 node = step(node);
-                if (node < 0 && node != kWalkDone && leaf == 0) {
-                    leaf = node;
+                if (node < 0 && node != kWalkDone && leaf2 == 0) {
+                    (leaf == 0 ? leaf : leaf2) = node;
                     node = st.empty() ? kWalkDone : st.pop();
                 }
                 if (st.all_lanes(leaf != 0)) break;
@@ -146,8 +146,9 @@ RTG_HD void bvh_walk(const SceneView &S, const Ray &r, float &tmax, Stack &st, V
                     st.on_prim();
                     if (visit(first + i, tmax)) return;
                 }
-                leaf = 0;
-                if (node < 0 && node != kWalkDone) {
+                leaf = leaf2;
+                leaf2 = 0;
+                if (leaf == 0 && node < 0 && node != kWalkDone) {
                     leaf = node;
                     node = st.empty() ? kWalkDone : st.pop();
                 }
