@@ -177,6 +177,54 @@ RTG_HD int emit_link(int child, const int *first, const int *last, const int *ne
     return newidx[child];
 }
 
+// Lay the top of the tree out first: the emitted nodes of the first BFS
+// levels from the root (at most kTopNodes) take emitted indices 0 ..
+// count - 1, in BFS order; the nodes that held those indices take the ones
+// the top nodes leave, so newidx stays a permutation of the emitted nodes.
+// Runs between the prefix sum of emitted nodes (newidx, non-decreasing over
+// the n_int radix-tree nodes) and the emit; needs more than kLeafMax
+// primitives (the root is emitted).  Returns the number of top nodes.
+RTG_HD int top_first(const int *left, const int *right, const int *first, const int *last, int n_int, int *newidx) {
+    auto emitted = [&](int v) { return v >= 0 && last[v] - first[v] + 1 > kLeafMax; };
+    int q[kTopNodes];
+    int nq = 0;
+    q[nq++] = 0;
+    for (int head = 0; head < nq && nq < kTopNodes; ++head) {
+        const int c0 = left[q[head]], c1 = right[q[head]];
+        if (emitted(c0) && nq < kTopNodes) q[nq++] = c0;
+        if (emitted(c1) && nq < kTopNodes) q[nq++] = c1;
+    }
+    // holder[j]: the emitted node whose emitted index is j (the first
+    // emitted v with newidx[v] == j)
+    int holder[kTopNodes], cur[kTopNodes];
+    for (int j = 0; j < nq; ++j) {
+        int lo = 0, hi = n_int - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (newidx[mid] < j)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        while (lo < n_int - 1 && !emitted(lo)) ++lo;
+        holder[j] = lo;
+        cur[j] = newidx[q[j]];
+    }
+    // each displaced holder takes an index a top node leaves (one outside
+    // 0 .. nq - 1 for every holder that is not a top node)
+    int k = 0;
+    for (int j = 0; j < nq; ++j) {
+        bool top = false;
+        for (int i = 0; i < nq; ++i) top = top || holder[j] == q[i];
+        if (top) continue;
+        while (k < nq && cur[k] < nq) ++k;
+        if (k >= nq) break;
+        newidx[holder[j]] = cur[k++];
+    }
+    for (int j = 0; j < nq; ++j) newidx[q[j]] = j;
+    return nq;
+}
+
 RTG_HD BvhNode make_node(const Box6 &b0, const Box6 &b1, int l0, int l1) {
     BvhNode n;
     n.xy0 = make_float4(b0.lo[0], b0.hi[0], b0.lo[1], b0.hi[1]);

@@ -3,6 +3,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "rt_scene.hpp"
+
 namespace rtg {
 
 constexpr int kBlockThreads = 128;  // two wave64s
@@ -16,6 +18,7 @@ struct DevStack {
     int *spill;
     int gstride;
     int sp;
+    const BvhNode *top;  // LDS copy of the nodes below kTopNodes (one-scene kernels), or null
     __device__ void reset() { sp = 0; }
     __device__ void push(int v) {
         if (sp < kLdsStack)
@@ -32,11 +35,36 @@ struct DevStack {
     __device__ void on_node() {}
     __device__ void on_prim() {}
     __device__ bool all_lanes(bool v) const { return __all(v); }  // over the wave's active lanes
+    __device__ BvhNode node(const SceneView &S, int i) const {
+#if defined(__HIP_DEVICE_COMPILE__)
+        typedef const BvhNode __attribute__((address_space(3))) LdsNode;  // `top` points into LDS
+        if (top != nullptr && i < kTopNodes) {
+            const BvhNode n = ((LdsNode *)top)[i];
+            // a compiler barrier (no instruction): keeps this LDS read from
+            // being merged with the HBM read below into one flat load
+            __asm__ volatile("" ::: "memory");
+            return n;
+        }
+#endif
+        return S.nodes[i];
+    }
 };
+
+// Copy the scene's top nodes into `dst` (a kTopNodes array in LDS) with the
+// whole block, then let `st` read them from there.  Every thread of the
+// block must call it (it synchronises the block).
+__device__ inline void stack_top_nodes(DevStack &st, BvhNode *dst, const SceneView &S) {
+    const int nt = min(kTopNodes, S.n_nodes);
+    const float4 *src = reinterpret_cast<const float4 *>(S.nodes);
+    float4 *d = reinterpret_cast<float4 *>(dst);
+    for (int i = threadIdx.x; i < 4 * nt; i += blockDim.x) d[i] = src[i];
+    __syncthreads();
+    st.top = dst;
+}
 
 __device__ inline DevStack make_stack(int *lds_base, int *spill) {
     return DevStack{lds_base + threadIdx.x, spill + (size_t)blockIdx.x * kBlockThreads + threadIdx.x,
-                    (int)gridDim.x * kBlockThreads, 0};
+                    (int)gridDim.x * kBlockThreads, 0, nullptr};
 }
 
 // Spill entries needed for a persistent grid of `blocks` blocks.

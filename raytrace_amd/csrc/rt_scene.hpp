@@ -26,6 +26,10 @@ struct alignas(16) BvhNode {
 };
 static_assert(sizeof(BvhNode) == 64, "64-byte node");
 
+// The nodes of the top five BFS levels are emitted first (indices <
+// kTopNodes): traversal kernels that render one scene keep them in LDS.
+constexpr int kTopNodes = 31;
+
 RTG_HD bool is_leaf(int c) { return c < 0; }
 RTG_HD int leaf_first(int c) { return (~c) >> 3; }
 RTG_HD int leaf_count(int c) { return ((~c) & 7) + 1; }
@@ -76,6 +80,7 @@ struct DevCamera {
 
 struct SceneView {
     const BvhNode *nodes;
+    int n_nodes;  // emitted BVH nodes
     const float4 *prims;
     int root;  // child encoding of the root; 0 with n_prims == 0 means empty
     uint32_t lights_on;  // bit i: light i is switched on (Light::bLight)

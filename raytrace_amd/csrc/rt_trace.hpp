@@ -70,6 +70,7 @@ struct ArrayStack {
     RTG_HD void on_node() {}
     RTG_HD void on_prim() {}
     RTG_HD bool all_lanes(bool v) const { return v; }  // one lane
+    RTG_HD BvhNode node(const SceneView &S, int i) const { return S.nodes[i]; }
 };
 
 constexpr int kWalkDone = (int)0x80000000;  // no node left: below every leaf code
@@ -96,7 +97,7 @@ RTG_HD void bvh_walk(const SceneView &S, const Ray &r, float &tmax, Stack &st, V
     int node = S.root;
     auto step = [&](int nd) -> int {  // one internal node: the next node, pushing the far child
         st.on_node();
-        const BvhNode &n = S.nodes[nd];
+        const BvhNode n = st.node(S, nd);
         const float4 a = n.xy0, b = n.xy1, z = n.z01;
         const int4 l = n.link;
         float t0, t1;

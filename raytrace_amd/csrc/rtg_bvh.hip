@@ -156,6 +156,12 @@ __global__ void k_reorder(const uint64_t *keys, int n, const float4 *src, float4
     for (int j = 0; j < 3; ++j) dst[3 * (size_t)k + j] = src[3 * (size_t)id + j];
 }
 
+// Top BFS levels first (top_first, rt_bvh.hpp); one thread.
+__global__ void k_top_first(const int *left, const int *right, const int *first, const int *last, int n_int,
+                            int *newidx) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) top_first(left, right, first, last, n_int, newidx);
+}
+
 __global__ void k_fill_i32(int *p, int n, int v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -332,6 +338,8 @@ int scene_build(SceneRes &S, const rtg_scene_desc &d, hipStream_t stream, SceneR
         RTG_HIP(hipMemcpyAsync(&last_flag, flags.as<int>() + (n - 2), sizeof(int), hipMemcpyDeviceToHost, stream));
         RTG_HIP(hipStreamSynchronize(stream));
         n_emit = last_new + last_flag;
+        k_top_first<<<1, 1, 0, stream>>>(left.as<int>(), right.as<int>(), first.as<int>(), last.as<int>(), n - 1,
+                                         newidx.as<int>());
         if (ensure(S.nodes, std::max(n_emit, 1) * sizeof(BvhNode))) return fail(RTG_E_NOMEM, "bvh nodes");
         k_emit<<<nblk(n - 1), kBlock, 0, stream>>>(sk, n - 1, boxes.as<Box6>(), left.as<int>(), right.as<int>(),
                                                    first.as<int>(), last.as<int>(), ibox.as<Box6>(), newidx.as<int>(),
@@ -362,6 +370,7 @@ int scene_build(SceneRes &S, const rtg_scene_desc &d, hipStream_t stream, SceneR
     SceneView &v = S.view;
     v = SceneView{};
     v.nodes = S.nodes.as<BvhNode>();
+    v.n_nodes = n_emit;
     v.prims = S.prims.as<float4>();
     v.root = root;
     v.lights_on = S.lights_on;

@@ -203,6 +203,10 @@ __global__ __launch_bounds__(kBlockThreads, 6) void k_extend(SceneView S0, WaveA
                                                              LineCtr *sched) {
     __shared__ int lds[kLdsStack * kBlockThreads];
     DevStack st = make_stack(lds, A.spill);
+    if constexpr (!B) {  // one scene: its top nodes in LDS
+        __shared__ BvhNode top_nodes[kTopNodes];
+        stack_top_nodes(st, top_nodes, S0);
+    }
     const int n = k == 0 ? A.n_samples : (int)*n_ptr;
     WaveQueue wq(sched, n, A.grab_lg);
     uint32_t c_prim = 0, c_refl = 0, c_refr = 0;
@@ -407,6 +411,10 @@ __global__ __launch_bounds__(kBlockThreads) void k_occlude(SceneView S0, WaveArg
                                                            const unsigned *shadow, LevelCtr *ctr, unsigned *masks) {
     __shared__ int lds[kLdsStack * kBlockThreads];
     DevStack st = make_stack(lds, A.spill);
+    if constexpr (!B) {  // one scene: its top nodes in LDS
+        __shared__ BvhNode top_nodes[kTopNodes];
+        stack_top_nodes(st, top_nodes, S0);
+    }
     const int n = (int)ctr->n_shadow.v;
     WaveQueue wq(ctr->sched[2], n, A.grab_lg);
     for (int base = wq.next(); base >= 0; base = wq.next()) {

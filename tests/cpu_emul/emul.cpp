@@ -43,6 +43,7 @@ struct TrackStack {
     int pop() { return d[--sp]; }
     bool empty() const { return sp == 0; }
     bool all_lanes(bool v) const { return v; }
+    BvhNode node(const SceneView &S, int i) const { return S.nodes[i]; }
 };
 
 struct HostScene {
@@ -54,6 +55,8 @@ struct HostScene {
     std::vector<uint8_t> tri_bins;
     SceneView view{};
     int depth = 0;
+    int n_top = 0;       // nodes laid out first (top BFS levels)
+    bool top_ok = true;  // emitted indices form a permutation with the root at 0
 };
 
 int build(const rtg_scene_desc &d, HostScene &H) {
@@ -123,6 +126,17 @@ int build(const rtg_scene_desc &d, HostScene &H) {
             newidx[i] = n_emit;
             n_emit += last[i] - first[i] + 1 > kLeafMax ? 1 : 0;
         }
+        // the device's layout: top BFS levels first; check it is a permutation
+        H.n_top = top_first(left.data(), right.data(), first.data(), last.data(), n - 1, newidx.data());
+        {
+            std::vector<int> seen(n_emit, 0);
+            bool ok = true;
+            for (int i = 0; i < n - 1; ++i) {
+                if (last[i] - first[i] + 1 <= kLeafMax) continue;
+                ok = ok && newidx[i] >= 0 && newidx[i] < n_emit && !seen[newidx[i]]++;
+            }
+            H.top_ok = ok && newidx[0] == 0;
+        }
         H.nodes.resize(std::max(n_emit, 1));
         for (int i = 0; i < n - 1; ++i) {
             if (last[i] - first[i] + 1 <= kLeafMax) continue;
@@ -140,6 +154,7 @@ int build(const rtg_scene_desc &d, HostScene &H) {
     }
     SceneView &v = H.view;
     v.nodes = H.nodes.data();
+    v.n_nodes = (int)H.nodes.size();
     v.prims = H.prims.data();
     v.root = root;
     v.n_prims = n;
@@ -248,6 +263,30 @@ int emul_render(const rtg_scene_desc *d, const rtg_params *p, int threads, const
     for (int i = 0; i < std::max(1, threads); ++i) th.emplace_back(work);
     for (auto &t : th) t.join();
     if (counts) counts[0] = c0, counts[1] = c1, counts[2] = c2, counts[3] = c3, counts[4] = c4;
+    return 0;
+}
+
+// BVH layout check of a scene: out = {emitted nodes, top nodes laid out
+// first, 1 if the emitted indices form a permutation with the root at 0,
+// 1 if the top nodes are the first BFS levels: every top node's emitted
+// children are top nodes or the top is full}.
+int emul_top_layout(const rtg_scene_desc *d, int32_t out[4]) {
+    HostScene H;
+    if (build(*d, H) != 0) return -1;
+    out[0] = H.view.n_nodes;
+    out[1] = H.n_top;
+    out[2] = H.top_ok ? 1 : 0;
+    // BFS property on the emitted tree: nodes 0 .. n_top - 1 are closed under
+    // "parent of" (a top node's parent is a top node), and the root is 0
+    bool ok = true;
+    std::vector<int> parent(H.nodes.size(), -1);
+    for (size_t i = 0; i < H.nodes.size(); ++i) {
+        const int4 l = H.nodes[i].link;
+        if (l.x >= 0) parent[l.x] = (int)i;
+        if (l.y >= 0) parent[l.y] = (int)i;
+    }
+    for (int i = 1; i < H.n_top && i < (int)H.nodes.size(); ++i) ok = ok && parent[i] >= 0 && parent[i] < H.n_top;
+    out[3] = ok ? 1 : 0;
     return 0;
 }
 
