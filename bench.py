@@ -137,9 +137,10 @@ def bench_serve(args) -> dict:
     rt.init(0)
     hs, ds = batch_scenes(rt)
     rates = [float(r) for r in args.rates.split(",")]
-    # warm-up, untimed: one short run of each mode (slot streams, workspaces,
-    # kernel code loading), as a server is warm before it takes traffic
-    warm = rt.gen_requests(64, 200.0, seed=40)
+    # warm-up, untimed: a full run of each mode at the first rate (slot
+    # streams, every slot's workspace grown to the request sizes, kernel code
+    # loading), as a server is warm before it takes traffic
+    warm = rt.gen_requests(max(8, int(rates[0] * args.duration)), rates[0], seed=40)
     warm_items = [(ds[s], rt.params(w, h, max_level=hs[s].info.max_level), t) for s, w, h, t in warm]
     for mode in (rt._capi.RTG_SERVE_BATCHED, rt._capi.RTG_SERVE_PLAIN):
         rt.serve(warm_items, mode=mode, slots=args.slots)
@@ -165,7 +166,7 @@ def bench_serve(args) -> dict:
         "data": "synthetic",
         "config": {"workload": f"Poisson arrivals at {args.rates} req/s, {args.duration:g} s each, requests 64x64.."
                                "1024x768 over 8 resident scenes, RGB to host", "slots": args.slots,
-                   "warmup": "one untimed run of each mode (64 requests at 200/s) before the sweep"},
+                   "warmup": "one untimed run of each mode at the first rate (other arrivals) before the sweep"},
         "rates": rows,
         "throughput_vs_plain_at_top_rate": round(top["batched"]["req_per_s"] / max(1e-9, top["plain"]["req_per_s"]), 2),
     }

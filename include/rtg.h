@@ -360,7 +360,8 @@ typedef struct rtg_serve_opts {
     int32_t mode;        /* RTG_SERVE_* */
     int32_t slots;       /* batches in flight at once, each on its own stream (0: 2) */
     int32_t max_batch;   /* requests per batch (0: 256, at most 4096) */
-    int32_t graph_after; /* capture a batch shape into a hipGraph at its n-th launch (0: 3; < 0: never) */
+    int32_t graph_after; /* capture a batch shape (of 8 or more requests) into a hipGraph at its n-th
+                            launch (0: 3; < 0: never) */
     uint64_t hbm_budget; /* queue bytes one batch may reserve (0: 4 GiB); admission stops there */
 } rtg_serve_opts;
 

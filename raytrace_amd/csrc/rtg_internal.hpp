@@ -194,6 +194,9 @@ int launch_megakernel(const SceneRes &S, const rtg_params &p, int W, int H, cons
                       Arena &A);
 // Launch one frame of the wavefront engine on `stream` (asynchronous) with
 // the buffers of `ws`; `cancel_host` (optional) is polled between levels.
+// Grow `ws` to what one W x H frame of the given depth, fanout and lights
+// reserves before it launches (nothing is allocated while it renders).
+int reserve_frame(Workspace &ws, int W, int H, int spp, int max_level, int fanout, int n_on, Arena &A);
 int launch_wavefront(const SceneRes &S, const rtg_params &p, int W, int H, const RenderOut &o, hipStream_t stream,
                      Workspace &ws, Arena &A, const int *cancel_host);
 

@@ -109,16 +109,26 @@ double raygen_tables(float fovy, int W, int H, float *tab) {
     return dp;
 }
 
+// The same tables written on the device: (x - W/2) * dp is one correctly
+// rounded double product, rounded to float, as on the host.
+__global__ void k_raygen_tables(float *tab, int W, int H, double dp) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < W + H; i += gridDim.x * blockDim.x)
+        tab[i] = i < W ? (float)((double)(i - W / 2) * dp) : (float)((double)(i - W - H / 2) * dp);
+}
+
 int upload_raygen_tables(const SceneRes &S, int W, int H, hipStream_t stream, Workspace &ws, double &dp) {
     dp = tan((double)S.camera.fovy * 3.1415926535897932384 / 360) / (H / 2);
     const uint32_t fb = __builtin_bit_cast(uint32_t, S.camera.fovy);
     const uint64_t key = (uint64_t)W | ((uint64_t)H << 16) | ((uint64_t)fb << 32);
     if (ws.tables.p != nullptr && ws.tables_key == key) return RTG_OK;
-    std::vector<float> tab((size_t)W + H);
-    dp = raygen_tables(S.camera.fovy, W, H, tab.data());
-    RTG_HIP(hipStreamSynchronize(stream));  // earlier frames may still read the old tables
-    if (ws.grow(ws.tables, tab.size() * sizeof(float))) return fail(RTG_E_NOMEM, "tables");
-    RTG_HIP(hipMemcpy(ws.tables.p, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
+    const size_t bytes = ((size_t)W + H) * sizeof(float);
+    if (ws.tables.p == nullptr || ws.tables.bytes < bytes) {
+        RTG_HIP(hipStreamSynchronize(stream));  // earlier frames may still read the old tables
+        if (ws.grow(ws.tables, bytes)) return fail(RTG_E_NOMEM, "tables");
+    }
+    // ordered after the earlier frames on `stream`, which read the old tables
+    k_raygen_tables<<<(W + H + 255) / 256, 256, 0, stream>>>(ws.tables.as<float>(), W, H, dp);
+    RTG_HIP(hipGetLastError());
     ws.tables_key = key;
     return RTG_OK;
 }

@@ -10,8 +10,9 @@
 // batch's RGB to pinned memory and an event.  Several slots run at once, so
 // a batch admitted while others are on the GPU overlaps them on the device.
 // A completed slot's RGB is copied into each request's buffer and the
-// request is done.  A batch shape launched `graph_after` times is captured
-// into a hipGraph and replayed from then on.
+// request is done.  A batch shape of at least kGraphMinBatch requests
+// launched `graph_after` times is captured into a hipGraph and replayed from
+// then on.
 //
 // Plain mode.  One slot, one request at a time, each rendered as its own
 // frame (the single-frame wavefront), in arrival order: the plain renderer
@@ -37,6 +38,8 @@ __global__ void k_zero_counts(unsigned long long *p) {
 }
 
 using Clock = std::chrono::steady_clock;
+
+constexpr int kGraphMinBatch = 8;  // smallest batch the loop captures into a hipGraph
 
 double since(Clock::time_point t0) { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 
@@ -106,9 +109,12 @@ static int slot_launch(ServeSlot &s, rtg_serve_request *reqs, const std::vector<
             items[k] = BatchItem{&q.scene->res, q.params, info[ids[k]].W, info[ids[k]].H,
                                  s.out.as<uint8_t>() + s.offs[k], nullptr, nullptr};
         }
+        // A small batch launches eagerly: capturing one of its many shapes costs
+        // milliseconds inside some request's latency for little launch time saved.
         int replayed = 0;
+        const int cap = (int)items.size() >= kGraphMinBatch ? capture_at : 0;
         st = launch_batch(items.data(), (int)items.size(), s.counts.as<unsigned long long>(),
-                          s.counts.as<unsigned long long>() + 4, s.stream, s.ws, A, capture_at, &replayed);
+                          s.counts.as<unsigned long long>() + 4, s.stream, s.ws, A, cap, &replayed);
     }
     if (st != RTG_OK) return st;
     RTG_HIP(hipMemcpyAsync(s.pinned, s.out.p, bytes, hipMemcpyDeviceToHost, s.stream));
@@ -143,8 +149,23 @@ int serve_run(rtg_serve_request *reqs, int n, const rtg_serve_opts &o, rtg_serve
         info[i].n_on = __builtin_popcount(q.scene->res.lights_on);
     }
     while ((int)A.serve_slots.size() < nslots) A.serve_slots.emplace_back(new ServeSlot());
+    // Size every slot for the largest single request before the clock starts,
+    // so that no request waits for a buffer to grow (a reallocation waits for
+    // the whole device).
+    int maxW = 1, maxH = 1, max_spp = 1, max_lvl = 0, max_fan = 0, max_on = 0;
+    size_t max_rgb = 0;
+    for (int i = 0; i < n; ++i) {
+        maxW = std::max(maxW, info[i].W), maxH = std::max(maxH, info[i].H);
+        max_spp = std::max(max_spp, reqs[i].params.spp > 0 ? (int)reqs[i].params.spp : 1);
+        max_lvl = std::max(max_lvl, (int)reqs[i].params.max_level);
+        max_fan = std::max(max_fan, info[i].fanout), max_on = std::max(max_on, info[i].n_on);
+        max_rgb = std::max(max_rgb, ((size_t)info[i].W * info[i].H * 3 + 255) & ~size_t(255));
+    }
     for (int k = 0; k < nslots; ++k) {
-        const int st = slot_init(*A.serve_slots[k]);
+        ServeSlot &s = *A.serve_slots[k];
+        int st = slot_init(s);
+        if (st == RTG_OK && n > 0) st = slot_reserve(s, max_rgb);
+        if (st == RTG_OK && n > 0) st = reserve_frame(s.ws, maxW, maxH, max_spp, max_lvl, max_fan, max_on, A);
         if (st != RTG_OK) return st;
     }
     std::vector<int> order(n);

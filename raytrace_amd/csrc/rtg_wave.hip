@@ -717,6 +717,17 @@ static int host_buffers(Workspace &ws) {
     return RTG_OK;
 }
 
+int reserve_frame(Workspace &ws, int W, int H, int spp, int max_level, int fanout, int n_on, Arena &A) {
+    int st = host_buffers(ws);
+    if (st != RTG_OK) return st;
+    if (ws.grow(ws.tables, ((size_t)W + H) * sizeof(float))) return fail(RTG_E_NOMEM, "tables");
+    const size_t samples = frame_samples(W, H, spp, A.tune.tile_lg);
+    // what launch_wavefront reserves up front: a no-sync plan of one pass
+    if (fanout >= 2 || samples > (size_t)kNoSyncMaxSamples) return RTG_OK;
+    const WavePlan pl = make_plan((int)samples, max_level, fanout, n_on, 1, A.no_sync_budget);
+    return pl.no_sync ? reserve_all(ws, pl, A.cus > 0 ? A.cus : 256) : RTG_OK;
+}
+
 int launch_wavefront(const SceneRes &S, const rtg_params &p, int W, int H, const RenderOut &o, hipStream_t stream,
                      Workspace &ws, Arena &A, const int *cancel_host) {
     double dp;
