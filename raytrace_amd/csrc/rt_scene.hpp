@@ -3,7 +3,8 @@
 // Layout, per scene:
 //   nodes    BVH2, 64 B per node: both children's boxes + links, read as four
 //            16-byte loads.  Children are internal node indices (>= 0) or
-//            leaves ~(first << 3 | (count - 1)) into `prims`.
+//            leaves ~(first << 3 | (count - 1)) into `prims`.  The nodes of
+//            the top five BFS levels come first (indices < kTopNodes).
 //   prims    48 B per primitive in leaf order (3 x float4):
 //              triangle: (p0w, gtri) (e1, obj) (e2, 0) — the reference's
 //                        clTri values p0 + position, p1 - p0, p2 - p0
