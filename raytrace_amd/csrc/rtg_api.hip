@@ -135,7 +135,25 @@ int Workspace::mark_done(hipStream_t s) {
     return RTG_OK;
 }
 
+int Workspace::side_init() {
+    if (side == nullptr) RTG_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    for (int k = 0; k < kLevels; ++k)
+        if (ev_fork[k] == nullptr) RTG_HIP(hipEventCreateWithFlags(&ev_fork[k], hipEventDisableTiming));
+    if (ev_join == nullptr) RTG_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    return RTG_OK;
+}
+
 void Workspace::release() {
+    if (side) (void)hipStreamSynchronize(side);
+    for (int k = 0; k < kLevels; ++k) {
+        if (ev_fork[k]) (void)hipEventDestroy(ev_fork[k]);
+        ev_fork[k] = nullptr;
+    }
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    ev_join = nullptr;
+    if (side) (void)hipStreamDestroy(side);
+    side = nullptr;
+    spill2.release();
     if (graph) (void)hipGraphExecDestroy(graph);
     graph = nullptr;
     if (done) (void)hipEventDestroy(done);

@@ -86,6 +86,13 @@ struct Workspace {
     hipEvent_t done = nullptr;
     int order_before(hipStream_t s);  // make `s` wait for the previous render
     int mark_done(hipStream_t s);     // record `done` on `s`
+    // Side stream of the level loop: the shadow rays of level k run on it
+    // while the render stream traces level k + 1 (own spill area: two
+    // traversal kernels at once).  Created on first use, default priority.
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork[kLevels] = {}, ev_join = nullptr;
+    DevBuf spill2;
+    int side_init();
     // Grow `b` to hold `bytes`; bumps the epoch if it moved.
     int grow(DevBuf &b, size_t bytes);
     void release();  // frees everything (the stream using it must be idle)

@@ -616,6 +616,8 @@ static int reserve_all(Workspace &ws, const WavePlan &pl, int cus) {
     const Grids<true> g1(cus);
     const int gmax = std::max(std::max(g0.extend, g0.occlude), std::max(g1.extend, g1.occlude));
     if (ws.grow(ws.spill, spill_ints(gmax) * sizeof(int))) return fail(RTG_E_NOMEM, "spill");
+    if (ws.grow(ws.spill2, spill_ints(std::max(g0.occlude, g1.occlude)) * sizeof(int)))
+        return fail(RTG_E_NOMEM, "spill");
     if (ws.grow(ws.w_qcount, (size_t)Workspace::kLevels * sizeof(LevelCtr))) return fail(RTG_E_NOMEM, "counters");
     if (ws.grow(ws.w_samples, (size_t)pl.n_samples * sizeof(float4))) return fail(RTG_E_NOMEM, "samples");
     for (int k = 0; k <= pl.max_level && pl.bound[k] > 0; ++k) {
@@ -630,7 +632,7 @@ static int reserve_all(Workspace &ws, const WavePlan &pl, int cus) {
 // allocates or waits.
 template <bool B>
 static int enqueue_wave(const SceneView &S0, const WaveArgs &a, const WavePlan &pl, hipStream_t stream,
-                        Workspace &ws, Arena &A, const int *cancel_host, bool retest) {
+                        Workspace &ws, Arena &A, const int *cancel_host, bool retest, bool overlap) {
     const int cus = A.cus > 0 ? A.cus : 256;
     const Grids<B> g(cus, A.tune.grid_pct);
     if (!pl.no_sync) {
@@ -639,6 +641,23 @@ static int enqueue_wave(const SceneView &S0, const WaveArgs &a, const WavePlan &
         if (ws.grow(ws.w_qcount, (size_t)Workspace::kLevels * sizeof(LevelCtr)))
             return fail(RTG_E_NOMEM, "queue counters");
     }
+    // The shadow rays of level k (k_fill_slots, k_occlude) depend only on
+    // k_shade(k), and level k + 1 does not depend on them: with `overlap`
+    // they run on the workspace's side stream, concurrently with the next
+    // level, and join the render stream before k_combine.
+    const bool ov = overlap && pl.n_on > 0;
+    if (ov) {
+        int st = ws.side_init();
+        if (st != RTG_OK) return st;
+        if (ws.grow(ws.spill2, spill_ints(g.occlude) * sizeof(int))) return fail(RTG_E_NOMEM, "spill");
+    }
+    hipStream_t so = ov ? ws.side : stream;
+    auto join = [&]() -> int {
+        if (!ov) return RTG_OK;
+        RTG_HIP(hipEventRecord(ws.ev_join, so));
+        RTG_HIP(hipStreamWaitEvent(stream, ws.ev_join, 0));
+        return RTG_OK;
+    };
     LevelCtr *ctr = ws.w_qcount.as<LevelCtr>();
     const int ctr_words = Workspace::kLevels * kCtrWords;
     k_zero_u32<<<(ctr_words + 255) / 256, 256, 0, stream>>>(ws.w_qcount.as<unsigned>(), ctr_words);
@@ -647,13 +666,18 @@ static int enqueue_wave(const SceneView &S0, const WaveArgs &a, const WavePlan &
     };
     WaveArgs args = a;
     args.spill = ws.spill.as<int>();
+    WaveArgs oargs = args;
+    if (ov) oargs.spill = ws.spill2.as<int>();
     size_t n_k = (size_t)pl.n_samples;
     int last = -1;
     for (int k = 0; k <= pl.max_level && n_k > 0; ++k) {
-        if (cancel_host && *cancel_host) return RTG_OK;
+        if (cancel_host && *cancel_host) return join();
         if (!pl.no_sync) {
             const int st = reserve_level(ws, pl, k, n_k);
-            if (st != RTG_OK) return st;
+            if (st != RTG_OK) {
+                (void)join();
+                return st;
+            }
         }
         const WRay *rays = k == 0 ? nullptr : ws.w_rays[k].as<WRay>();
         const unsigned *n_ptr = k == 0 ? nullptr : &ctr[k - 1].n_next.v;
@@ -671,13 +695,17 @@ static int enqueue_wave(const SceneView &S0, const WaveArgs &a, const WavePlan &
             S0, args, k, rays, n_ptr, ws.w_hits[k].as<WHit>(), ws.w_cs[k].as<int4>(), ws.w_nodes[k].as<WNode>(),
             ws.w_masks[k].as<unsigned>(), next, ctr + k, ws.w_shadow[k].as<unsigned>());
         ktime_end(A, stream, kt, n_ptr);
-        k_fill_slots<<<256, 256, 0, stream>>>(ws.w_slots[k].as<float4>(), n_ptr, k == 0 ? pl.n_samples : -1);
+        if (ov) {
+            RTG_HIP(hipEventRecord(ws.ev_fork[k], stream));
+            RTG_HIP(hipStreamWaitEvent(so, ws.ev_fork[k], 0));
+        }
+        k_fill_slots<<<256, 256, 0, so>>>(ws.w_slots[k].as<float4>(), n_ptr, k == 0 ? pl.n_samples : -1);
         if (pl.n_on > 0) {
-            kt = ktime_begin(A, stream, KT_OCCLUDE, 0);
-            k_occlude<B><<<grid(g.occlude, n_k * pl.n_on), kBlockThreads, 0, stream>>>(
-                S0, args, ws.w_nodes[k].as<WNode>(), ws.w_shadow[k].as<unsigned>(), ctr + k,
+            kt = ktime_begin(A, so, KT_OCCLUDE, 0);
+            k_occlude<B><<<grid(g.occlude, n_k * pl.n_on), kBlockThreads, 0, so>>>(
+                S0, oargs, ws.w_nodes[k].as<WNode>(), ws.w_shadow[k].as<unsigned>(), ctr + k,
                 ws.w_masks[k].as<unsigned>());
-            ktime_end(A, stream, kt, &ctr[k].n_shadow.v);
+            ktime_end(A, so, kt, &ctr[k].n_shadow.v);
         }
         RTG_HIP(hipGetLastError());
         last = k;
@@ -689,6 +717,10 @@ static int enqueue_wave(const SceneView &S0, const WaveArgs &a, const WavePlan &
         RTG_HIP(hipMemcpyAsync(ws.qcount_host, &ctr[k].n_next.v, sizeof(unsigned), hipMemcpyDeviceToHost, stream));
         RTG_HIP(hipStreamSynchronize(stream));
         n_k = ws.qcount_host[0];
+    }
+    {
+        const int st = join();
+        if (st != RTG_OK) return st;
     }
     if (cancel_host && *cancel_host) return RTG_OK;
     if (!pl.no_sync && ws.grow(ws.w_samples, (size_t)pl.n_samples * sizeof(float4)))
@@ -719,8 +751,11 @@ static int host_buffers(Workspace &ws) {
 
 int reserve_frame(Workspace &ws, int W, int H, int spp, int max_level, int fanout, int n_on, Arena &A) {
     int st = host_buffers(ws);
+    if (st == RTG_OK) st = ws.side_init();
     if (st != RTG_OK) return st;
     if (ws.grow(ws.tables, ((size_t)W + H) * sizeof(float))) return fail(RTG_E_NOMEM, "tables");
+    if (ws.grow(ws.spill2, spill_ints(Grids<false>(A.cus > 0 ? A.cus : 256).occlude) * sizeof(int)))
+        return fail(RTG_E_NOMEM, "spill");
     const size_t samples = frame_samples(W, H, spp, A.tune.tile_lg);
     // what launch_wavefront reserves up front: a no-sync plan of one pass
     if (fanout >= 2 || samples > (size_t)kNoSyncMaxSamples) return RTG_OK;
@@ -768,7 +803,7 @@ int launch_wavefront(const SceneRes &S, const rtg_params &p, int W, int H, const
         // against the worst-case bounds.
         if (r.n_samples > kNoSyncMaxSamples) pl.no_sync = false;
         if (pl.no_sync && (st = reserve_all(ws, pl, A.cus > 0 ? A.cus : 256)) != RTG_OK) return st;
-        st = enqueue_wave<false>(S.view, a, pl, stream, ws, A, cancel_host, r.retest != 0);
+        st = enqueue_wave<false>(S.view, a, pl, stream, ws, A, cancel_host, r.retest != 0, true);
         if (st != RTG_OK || (cancel_host && *cancel_host)) return st;
     }
     return RTG_OK;
@@ -889,10 +924,12 @@ int launch_batch(const BatchItem *items, int n, unsigned long long *counts, unsi
                          (ws.epoch * 0x9E3779B97F4A7C15ull) ^ ((uint64_t)(uintptr_t)counts * 0xC2B2AE3D27D4EB4Full) ^
                          ((uint64_t)(uintptr_t)retests * 0x165667B19E3779F9ull) ^
                          (A.tune.key() * 0x27D4EB2F165667C5ull);
-    auto enqueue = [&]() -> int {
+    // eager launches overlap each level's shadow rays with the next level on
+    // the side stream; a captured graph keeps the one-stream order
+    auto enqueue = [&](bool overlap) -> int {
         k_chunk_req<<<(int)((n_chunks + 255) / 256), 256, 0, stream>>>(a.reqs, n_pad, ws.b_chunks.as<int>(),
                                                                       (int)n_chunks);
-        return enqueue_wave<true>(SceneView{}, a, pl, stream, ws, A, nullptr, any_retest);
+        return enqueue_wave<true>(SceneView{}, a, pl, stream, ws, A, nullptr, any_retest, overlap);
     };
     if (use_graph && capture_at > 1 && !(ws.graph != nullptr && ws.graph_key == key)) {
         int slot = -1;
@@ -905,14 +942,14 @@ int launch_batch(const BatchItem *items, int n, unsigned long long *counts, unsi
         }
         if ((int)++ws.seen_count[slot] < capture_at) use_graph = false;
     }
-    if (!use_graph) return enqueue();
+    if (!use_graph) return enqueue(true);
     if (ws.graph == nullptr || ws.graph_key != key) {
         if (ws.graph) (void)hipGraphExecDestroy(ws.graph);
         ws.graph = nullptr;
         const bool kt = A.ktimes;
         A.ktimes = false;  // no timing events inside a captured graph
         RTG_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        st = enqueue();
+        st = enqueue(false);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(stream, &g);
         A.ktimes = kt;
