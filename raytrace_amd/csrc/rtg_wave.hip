@@ -924,8 +924,8 @@ int launch_batch(const BatchItem *items, int n, unsigned long long *counts, unsi
                          (ws.epoch * 0x9E3779B97F4A7C15ull) ^ ((uint64_t)(uintptr_t)counts * 0xC2B2AE3D27D4EB4Full) ^
                          ((uint64_t)(uintptr_t)retests * 0x165667B19E3779F9ull) ^
                          (A.tune.key() * 0x27D4EB2F165667C5ull);
-    // eager launches overlap each level's shadow rays with the next level on
-    // the side stream; a captured graph keeps the one-stream order
+    // each level's shadow rays beside the next level on the side stream
+    // (`overlap`), or all on `stream`
     auto enqueue = [&](bool overlap) -> int {
         k_chunk_req<<<(int)((n_chunks + 255) / 256), 256, 0, stream>>>(a.reqs, n_pad, ws.b_chunks.as<int>(),
                                                                       (int)n_chunks);
@@ -948,22 +948,34 @@ int launch_batch(const BatchItem *items, int n, unsigned long long *counts, unsi
         ws.graph = nullptr;
         const bool kt = A.ktimes;
         A.ktimes = false;  // no timing events inside a captured graph
-        RTG_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        st = enqueue(false);
-        hipGraph_t g = nullptr;
-        const hipError_t e = hipStreamEndCapture(stream, &g);
+        // Capture with the side stream (the fork and join events become graph
+        // edges); should that capture fail, capture the one-stream order.
+        std::string why;
+        for (int attempt = 0; attempt < 2 && ws.graph == nullptr; ++attempt) {
+            RTG_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            st = enqueue(attempt == 0);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(stream, &g);
+            if (st != RTG_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                A.ktimes = kt;
+                return st;
+            }
+            if (e != hipSuccess) {
+                why = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
+                (void)hipGetLastError();
+                continue;
+            }
+            const hipError_t ei = hipGraphInstantiate(&ws.graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ei != hipSuccess) {
+                ws.graph = nullptr;
+                why = std::string("hipGraphInstantiate: ") + hipGetErrorString(ei);
+                (void)hipGetLastError();
+            }
+        }
         A.ktimes = kt;
-        if (st != RTG_OK) {
-            if (g) (void)hipGraphDestroy(g);
-            return st;
-        }
-        if (e != hipSuccess) return fail(RTG_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-        const hipError_t ei = hipGraphInstantiate(&ws.graph, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (ei != hipSuccess) {
-            ws.graph = nullptr;
-            return fail(RTG_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
-        }
+        if (ws.graph == nullptr) return fail(RTG_E_DEVICE, why);
         ws.graph_key = key;
     } else if (graph_replayed) {
         *graph_replayed = 1;
