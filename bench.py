@@ -173,8 +173,11 @@ def bench_serve(args) -> dict:
         sys.stderr.write(f"rate {rate:g}/s batched: {cap_rows[-1]['batched']}\n")
         sys.stderr.flush()
     top = rows[-1]
-    best_b = max((r["batched"] for r in rows + cap_rows), key=lambda b: b["req_per_s"])
+    # the plain renderer's highest throughput, and the batched loop's highest
+    # throughput at a p99 no worse than the plain renderer's there
     best_p = max((r["plain"] for r in rows), key=lambda b: b["req_per_s"])
+    ok_b = [r["batched"] for r in rows + cap_rows if r["batched"]["p99_ms"] <= best_p["p99_ms"]]
+    best_b = max(ok_b or [top["batched"]], key=lambda b: b["req_per_s"])
     return {
         "metric": "serve-loop p50/p99 frame latency and throughput", "value": top["batched"]["req_per_s"],
         "unit": "requests/s at the highest offered rate of the sweep", "n_gpus": 1, "steps": len(rates), "warmup": 0,
@@ -187,7 +190,7 @@ def bench_serve(args) -> dict:
         "rates": rows,
         "capacity": cap_rows,
         "throughput_vs_plain_at_top_rate": round(top["batched"]["req_per_s"] / max(1e-9, top["plain"]["req_per_s"]), 2),
-        "max_throughput": {"batched": best_b, "plain": best_p,
+        "max_throughput": {"plain": best_p, "batched_at_no_worse_p99": best_b,
                            "ratio": round(best_b["req_per_s"] / max(1e-9, best_p["req_per_s"]), 2)},
     }
 
