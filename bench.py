@@ -282,6 +282,8 @@ def main() -> int:
                     help="workload 4: offered rates beyond the sweep, batched loop only (its throughput ceiling)")
     ap.add_argument("--no-autotune", action="store_true", help="workload 3: keep the default launch tuning")
     ap.add_argument("--autotune", action="store_true", help="workload 2: autotune the launch tuning first")
+    ap.add_argument("--one-stream", action="store_true",
+                    help="workload 2: every kernel on the render stream (RTG_FLAG_ONE_STREAM), for counter collection")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (reference for FETCH_SIZE / WRITE_SIZE scaling)")
     args = ap.parse_args()
@@ -322,7 +324,8 @@ def main() -> int:
     hs = rt.HostScene.workload(2)
     ds = rt.DeviceScene(hs)
     build_ms = ds.build_ms
-    p = rt.params(W, H, max_level=DEPTH, engine=engine)
+    flags = _capi.RTG_FLAG_ONE_STREAM if args.one_stream else 0
+    p = rt.params(W, H, max_level=DEPTH, engine=engine, flags=flags)
 
     dev = torch.device("cuda:0")
     rgb = torch.empty((H, W, 3), dtype=torch.uint8, device=dev)
@@ -364,7 +367,7 @@ def main() -> int:
 
     # Per-kernel device time: the same K frames again, with HIP events around
     # every launch on the render stream (RTG_FLAG_KERNEL_TIMES).
-    pk = rt.params(W, H, max_level=DEPTH, engine=engine, flags=rt.RTG_FLAG_KERNEL_TIMES)
+    pk = rt.params(W, H, max_level=DEPTH, engine=engine, flags=rt.RTG_FLAG_KERNEL_TIMES | flags)
     rt.kernel_times()
     for _ in range(args.steps):
         rc = lib.rtg_render_device(ds.h, C.byref(pk), C.c_void_p(rgb.data_ptr()), None, None,
@@ -440,7 +443,8 @@ def main() -> int:
         "config": {"workload": "1920x1080 1spp depth 4, 200k-tri displaced icosphere + 64 spheres, 4 point lights",
                    "engine": {"mega": "megakernel", "wave": "wavefront"}[eng_name], "rays_per_frame": rays_per_frame,
                    "rays_by_type": dict(zip(["primary", "shadow", "reflect", "refract"], per_type)),
-                   "bvh_build_ms": round(build_ms, 3), "tuning": rt.get_tuning()},
+                   "bvh_build_ms": round(build_ms, 3), "tuning": rt.get_tuning(),
+                   "side_stream": not args.one_stream},
         "roofline": roofline, "cpu_baseline": cpu,
     }
     if tune is not None:

@@ -73,6 +73,9 @@ extern "C" {
 #define RTG_FLAG_FP64_RETEST 0x1 /* re-test near-tie primary candidates in FP64 */
 #define RTG_FLAG_KERNEL_TIMES 0x2 /* time every kernel launch of the render (see rtg_kernel_times) */
 #define RTG_FLAG_EAGER 0x4        /* batch: launch kernel by kernel, no captured graph */
+#define RTG_FLAG_ONE_STREAM 0x8   /* run every kernel of the frame on the render stream: no side stream for
+                                     the shadow rays (callers that must not use a second stream; counter
+                                     collection that serialises dispatches) */
 
 /* Hard limits of this build (the reference's own caps are noted beside). */
 #define RTG_MAX_LIGHTS 32      /* reference AddLight caps at 8 (Scene.cpp:85-86) */

@@ -28,6 +28,7 @@ RTG_ENGINE_WAVEFRONT = 2
 RTG_FLAG_FP64_RETEST = 0x1
 RTG_FLAG_KERNEL_TIMES = 0x2
 RTG_FLAG_EAGER = 0x4
+RTG_FLAG_ONE_STREAM = 0x8
 RTG_SERVE_BATCHED = 0
 RTG_SERVE_PLAIN = 1
 

@@ -12,7 +12,12 @@ namespace rtg {
 namespace {
 thread_local std::string g_err = "";
 std::mutex g_init_lock;
-std::unique_ptr<Arena> g_arena;
+// The device state of rtg_init.  Owned by rtg_init / rtg_shutdown only: it is
+// never destroyed by a static destructor at process exit, where the HIP
+// runtime (or a profiler attached to it) may already be torn down and a
+// stream or event call aborts; the driver reclaims the memory with the
+// process.
+Arena *g_arena = nullptr;
 
 
 uint64_t hash_bytes(uint64_t h, const void *p, size_t n) {
@@ -172,7 +177,7 @@ void Workspace::release() {
     ++epoch;
 }
 
-Arena *arena() { return g_arena.get(); }
+Arena *arena() { return g_arena; }
 
 static hipEvent_t take_event(Arena &A) {
     if (!A.kevents.empty()) {
@@ -280,7 +285,7 @@ int rtg_init(int device) {
     RTG_HIP(hipHostGetDevicePointer(reinterpret_cast<void **>(&A->cancel_dev), A->cancel_host, 0));
     // ray counts [0..3] (primary, shadow, reflection, refraction), FP64 re-tests [4]
     if (ensure(A->counters, 8 * sizeof(unsigned long long))) return fail(RTG_E_NOMEM, "counters");
-    g_arena = std::move(A);
+    g_arena = A.release();
     return RTG_OK;
 }
 
@@ -307,7 +312,8 @@ int rtg_shutdown(void) {
         (void)hipEventDestroy(A.ev1);
         (void)hipStreamDestroy(A.stream);
     }
-    g_arena.reset();
+    delete g_arena;
+    g_arena = nullptr;
     return RTG_OK;
 }
 

@@ -803,7 +803,8 @@ int launch_wavefront(const SceneRes &S, const rtg_params &p, int W, int H, const
         // against the worst-case bounds.
         if (r.n_samples > kNoSyncMaxSamples) pl.no_sync = false;
         if (pl.no_sync && (st = reserve_all(ws, pl, A.cus > 0 ? A.cus : 256)) != RTG_OK) return st;
-        st = enqueue_wave<false>(S.view, a, pl, stream, ws, A, cancel_host, r.retest != 0, true);
+        st = enqueue_wave<false>(S.view, a, pl, stream, ws, A, cancel_host, r.retest != 0,
+                                 (p.flags & RTG_FLAG_ONE_STREAM) == 0);
         if (st != RTG_OK || (cancel_host && *cancel_host)) return st;
     }
     return RTG_OK;
@@ -848,7 +849,7 @@ int launch_batch(const BatchItem *items, int n, unsigned long long *counts, unsi
     while (n_pad < n) n_pad <<= 1;
     std::vector<WaveReq> reqs(n_pad);
     size_t total = 0;
-    bool any_retest = false;
+    bool any_retest = false, one_stream = false;
     for (int i = 0; i < n; ++i) {
         const BatchItem &it = items[i];
         WaveReq &r = reqs[i];
@@ -864,6 +865,7 @@ int launch_batch(const BatchItem *items, int n, unsigned long long *counts, unsi
         r.retest = (it.params.flags & RTG_FLAG_FP64_RETEST) != 0;
         r.no_refract = it.params.mode == RTG_MODE_REFLECTTEST;
         any_retest = any_retest || r.retest;
+        one_stream = one_stream || (it.params.flags & RTG_FLAG_ONE_STREAM) != 0;
         r.rgb = it.rgb, r.ids = it.ids, r.tprim = it.tprim;
         total += (size_t)r.n_samples;
     }
@@ -929,7 +931,7 @@ int launch_batch(const BatchItem *items, int n, unsigned long long *counts, unsi
     auto enqueue = [&](bool overlap) -> int {
         k_chunk_req<<<(int)((n_chunks + 255) / 256), 256, 0, stream>>>(a.reqs, n_pad, ws.b_chunks.as<int>(),
                                                                       (int)n_chunks);
-        return enqueue_wave<true>(SceneView{}, a, pl, stream, ws, A, nullptr, any_retest, overlap);
+        return enqueue_wave<true>(SceneView{}, a, pl, stream, ws, A, nullptr, any_retest, overlap && !one_stream);
     };
     if (use_graph && capture_at > 1 && !(ws.graph != nullptr && ws.graph_key == key)) {
         int slot = -1;
