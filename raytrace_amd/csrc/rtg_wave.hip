@@ -530,8 +530,10 @@ __global__ void k_chunk_req(const WaveReq *reqs, int n_reqs, int *chunk_req, int
     chunk_req[c] = r;
 }
 
-// Single frames with more samples run level by level (see launch_wavefront).
-constexpr int kNoSyncMaxSamples = 1 << 20;
+// Single frames with more samples run level by level (see launch_wavefront):
+// 4M covers 1080p at 2 spp; the persistent kernels read each level's real
+// count on the device, so a no-sync frame only pays for its queue memory.
+constexpr int kNoSyncMaxSamples = 1 << 22;
 // Level-0 samples per pass of a frame whose rays branch (see launch_wavefront).
 constexpr int kPassSamples = 1 << 21;
 
@@ -798,9 +800,9 @@ int launch_wavefront(const SceneRes &S, const rtg_params &p, int W, int H, const
         a.n_samples = r.n_samples;
         WavePlan pl = make_plan(r.n_samples, p.max_level, S.max_fanout, __builtin_popcount(S.lights_on), 1,
                                 A.no_sync_budget);
-        // A large frame keeps the GPU busy at every level; there, sizing each
-        // level from the previous level's count beats launching full grids
-        // against the worst-case bounds.
+        // A very large frame keeps the GPU busy at every level; there the
+        // host round trip per level costs little, and sizing each level from
+        // the previous level's count saves the worst-case queue memory.
         if (r.n_samples > kNoSyncMaxSamples) pl.no_sync = false;
         if (pl.no_sync && (st = reserve_all(ws, pl, A.cus > 0 ? A.cus : 256)) != RTG_OK) return st;
         st = enqueue_wave<false>(S.view, a, pl, stream, ws, A, cancel_host, r.retest != 0,
