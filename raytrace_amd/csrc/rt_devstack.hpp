@@ -20,16 +20,35 @@ struct DevStack {
     int sp;
     const BvhNode *top;  // LDS copy of the nodes below kTopNodes (one-scene kernels), or null
     __device__ void reset() { sp = 0; }
+    // The LDS part is addressed as LDS (ds_read / ds_write); the compiler
+    // barriers keep each LDS access from being merged with the spill access
+    // of the other branch into one flat access, which would take the vector
+    // memory path and its latency on every push and pop.
     __device__ void push(int v) {
-        if (sp < kLdsStack)
-            lds[sp * kBlockThreads] = v;
-        else
+#if defined(__HIP_DEVICE_COMPILE__)
+        typedef int __attribute__((address_space(3))) LdsInt;
+        if (sp < kLdsStack) {
+            ((LdsInt *)lds)[sp * kBlockThreads] = v;
+            __asm__ volatile("" ::: "memory");
+        } else {
             spill[(sp - kLdsStack) * gstride] = v;
+        }
+#endif
         ++sp;
     }
     __device__ int pop() {
         --sp;
-        return sp < kLdsStack ? lds[sp * kBlockThreads] : spill[(sp - kLdsStack) * gstride];
+        int v = 0;
+#if defined(__HIP_DEVICE_COMPILE__)
+        typedef const int __attribute__((address_space(3))) LdsInt;
+        if (sp < kLdsStack) {
+            v = ((LdsInt *)lds)[sp * kBlockThreads];
+            __asm__ volatile("" ::: "memory");
+        } else {
+            v = spill[(sp - kLdsStack) * gstride];
+        }
+#endif
+        return v;
     }
     __device__ bool empty() const { return sp == 0; }
     __device__ void on_node() {}

@@ -341,6 +341,24 @@ RTG_HD bool retest64(const SceneView &S, const TraceRay &q, int pi, double &t64,
     return true;
 }
 
+// The three 16-B parts of BVH primitive slot pi, read together: one memory
+// round trip per primitive test instead of a read of the tag followed by the
+// reads its branch needs (a sphere leaves the third part unused).
+struct PrimRec {
+    float4 a, b, c;
+};
+RTG_HD PrimRec load_prim(const SceneView &S, int pi) {
+    const float4 *p = S.prims + 3 * (size_t)pi;
+    PrimRec r{p[0], p[1], p[2]};
+#if defined(__HIP_DEVICE_COMPILE__)
+    // uses every component here (no instruction), so the three reads are
+    // issued as whole 16-B loads before the tag decides the branch
+    __asm__ volatile("" ::"v"(r.a.x), "v"(r.a.y), "v"(r.a.z), "v"(r.a.w), "v"(r.b.x), "v"(r.b.y), "v"(r.b.z),
+                     "v"(r.b.w), "v"(r.c.x), "v"(r.c.y), "v"(r.c.z), "v"(r.c.w));
+#endif
+    return r;
+}
+
 // Closest hit (RayTracer.cpp:455-465).  kRetest: FP64 near-tie re-test (see
 // above); *retested counts the queries that were re-tested.  kBefore: only
 // objects with index < `before` take part (see child_src).
@@ -366,11 +384,12 @@ RTG_HD Hit trace_closest(const SceneView &S, const TraceRay &q, Stack &st, int *
         }
     };
     bvh_walk<!kRetest>(S, q.r, tmax, st, [&](int pi, float &tm) -> bool {
-        const float4 A = S.prims[3 * pi];
+        const PrimRec R = load_prim(S, pi);
+        const float4 A = R.a;
         const int tag = as_int(A.w);
         const V3 p = V3{A.x, A.y, A.z};
         if (tag >= 0) {  // triangle, tag = global index
-            const float4 B = S.prims[3 * pi + 1], C = S.prims[3 * pi + 2];
+            const float4 B = R.b, C = R.c;
             const int obj = as_int(B.w);
             if (kBefore && obj >= before) return false;
             if (tag == q.src.prim && obj == q.src.obj && !q.self_test) return false;
@@ -401,7 +420,7 @@ RTG_HD Hit trace_closest(const SceneView &S, const TraceRay &q, Stack &st, int *
         } else {  // sphere, tag = ~object
             const int obj = ~tag;
             if (kBefore && obj >= before) return false;
-            const float4 B = S.prims[3 * pi + 1];
+            const float4 B = R.b;
             float t;
             if (q.src.obj == obj) {
                 if (!q.inside) return false;
@@ -480,19 +499,20 @@ RTG_HD bool trace_any(const SceneView &S, const TraceRay &q, float dis, Stack &s
     bool blocked = false;
     float tmax = dis;
     bvh_walk<true>(S, q.r, tmax, st, [&](int pi, float &) -> bool {
-        const float4 A = S.prims[3 * pi];
+        const PrimRec R = load_prim(S, pi);
+        const float4 A = R.a;
         const int tag = as_int(A.w);
         const V3 p = V3{A.x, A.y, A.z};
         float t;
         if (tag >= 0) {
-            const float4 B = S.prims[3 * pi + 1], C = S.prims[3 * pi + 2];
+            const float4 B = R.b, C = R.c;
             const int obj = as_int(B.w);
             if (tag == q.src.prim && obj == q.src.obj && !q.self_test) return false;
             float w, u, v;
             t = tri_test(q.r, V3{B.x, B.y, B.z}, V3{C.x, C.y, C.z}, p, w, u, v);
         } else {
             const int obj = ~tag;
-            const float4 B = S.prims[3 * pi + 1];
+            const float4 B = R.b;
             if (q.src.obj == obj) {
                 if (!q.inside) return false;
                 blocked = B.y < dis;  // HitRes(radius) (Basic3DObject.cpp:143-144)

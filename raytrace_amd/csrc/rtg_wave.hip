@@ -407,7 +407,7 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_shade(SceneView S0, WaveAr
 }
 
 template <bool B>
-__global__ __launch_bounds__(kBlockThreads) void k_occlude(SceneView S0, WaveArgs A, const WNode *nodes,
+__global__ __launch_bounds__(kBlockThreads, 8) void k_occlude(SceneView S0, WaveArgs A, const WNode *nodes,
                                                            const unsigned *shadow, LevelCtr *ctr, unsigned *masks) {
     __shared__ int lds[kLdsStack * kBlockThreads];
     DevStack st = make_stack(lds, A.spill);
