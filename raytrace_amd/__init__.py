@@ -178,11 +178,19 @@ class DeviceScene:
         check(self._lib.rtg_scene_bytes(self.h, C.byref(v)), "rtg_scene_bytes")
         return v.value
 
-    def render(self, p: rtg_params, ids: bool = True, tprim: bool = True) -> dict:
+    def render(self, p: rtg_params, ids: bool = True, tprim: bool = True, out: dict | None = None) -> dict:
+        """rtg_render into new arrays, or into the arrays of `out` (a previous
+        result of the same shape, reused: no fresh host pages per frame)."""
         W, H = p.width, p.height
-        rgb = np.empty((H, W, 3), np.uint8)
-        idb = np.empty((H, W, 2), np.int32) if ids else None
-        tb = np.empty((H, W), np.float32) if tprim else None
+        if out is not None:
+            rgb, idb, tb = out["rgb"], out["ids"] if ids else None, out["t"] if tprim else None
+            _check_out(rgb, (H, W, 3), np.uint8, "rgb")
+            _check_out(idb, (H, W, 2), np.int32, "ids", ids)
+            _check_out(tb, (H, W), np.float32, "t", tprim)
+        else:
+            rgb = np.empty((H, W, 3), np.uint8)
+            idb = np.empty((H, W, 2), np.int32) if ids else None
+            tb = np.empty((H, W), np.float32) if tprim else None
         st = rtg_stats()
         ptr = lambda a: a.ctypes.data_as(C.c_void_p) if a is not None else None
         check(self._lib.rtg_render(self.h, C.byref(p), ptr(rgb), ptr(idb), ptr(tb), C.byref(st)), "rtg_render")
@@ -211,11 +219,22 @@ def kernel_times() -> dict:
 
 
 
-def render_batch(items, ids: bool = True, eager: bool = False):
+def _check_out(a, shape, dtype, what: str, wanted: bool = True) -> None:
+    if not wanted:
+        return
+    if a is None or a.shape != shape or a.dtype != dtype or not a.flags.c_contiguous or not a.flags.writeable:
+        raise ValueError(f"out[{what!r}] must be a writeable C-contiguous {np.dtype(dtype).name} array of shape {shape}")
+
+
+def render_batch(items, ids: bool = True, eager: bool = False, out: list | None = None):
     """rtg_render_batch over [(DeviceScene, rtg_params), ...]: one wavefront for
-    all requests.  Returns ([{"rgb", "ids"} per request], rtg_stats)."""
+    all requests.  Returns ([{"rgb", "ids"} per request], rtg_stats).  `out`: the
+    result list of a previous batch of the same shapes, whose arrays are
+    reused (no fresh host pages per batch)."""
     lib = _capi.load()
     n = len(items)
+    if out is not None and len(out) != n:
+        raise ValueError(f"out holds {len(out)} results for {n} requests")
     reqs = (_capi.rtg_request * max(1, n))()
     outs = []
     for k, (ds, p) in enumerate(items):
@@ -223,8 +242,13 @@ def render_batch(items, ids: bool = True, eager: bool = False):
         C.memmove(C.byref(q), C.byref(p), C.sizeof(rtg_params))
         if eager:
             q.flags |= _capi.RTG_FLAG_EAGER
-        rgb = np.empty((p.height, p.width, 3), np.uint8)
-        idb = np.empty((p.height, p.width, 2), np.int32) if ids else None
+        if out is not None:
+            rgb, idb = out[k]["rgb"], out[k]["ids"] if ids else None
+            _check_out(rgb, (p.height, p.width, 3), np.uint8, "rgb")
+            _check_out(idb, (p.height, p.width, 2), np.int32, "ids", ids)
+        else:
+            rgb = np.empty((p.height, p.width, 3), np.uint8)
+            idb = np.empty((p.height, p.width, 2), np.int32) if ids else None
         reqs[k].scene = ds.h
         reqs[k].params = q
         reqs[k].rgb = rgb.ctypes.data_as(C.POINTER(C.c_uint8))

@@ -74,6 +74,14 @@ def test_batch_graph_replays_match_eager(rt_gpu):
         for k, (a, b) in enumerate(zip(again, eager)):
             assert (a["rgb"] == b["rgb"]).all() and (a["ids"] == b["ids"]).all(), f"replay {rep}, request {k}"
         assert st.rays_total == st_e.rays_total
+    # reused host buffers, scribbled first: the replay overwrites every pixel
+    for o in first:
+        o["rgb"].fill(7)
+        o["ids"].fill(-7)
+    again, st = rt.render_batch(items, out=first)
+    assert st.graph_replays == 1 and again[0]["rgb"] is first[0]["rgb"]
+    for k, (a, b) in enumerate(zip(again, eager)):
+        assert (a["rgb"] == b["rgb"]).all() and (a["ids"] == b["ids"]).all(), f"reused buffers, request {k}"
     # same padded shape, different content: the replay reads the new tables
     items2 = [(d, rt.params(p.width, p.height, max_level=p.max_level, spp=p.spp, seed=p.seed + 100,
                             tile_compat=p.tile_compat)) for d, p in reversed(items)]
@@ -93,5 +101,10 @@ def test_batch_single_request_and_empty(rt_gpu):
     outs, st = rt.render_batch([(ds, p)])
     ref = ds.render(p)
     assert (outs[0]["rgb"] == ref["rgb"]).all() and (outs[0]["ids"] == ref["ids"]).all()
+    # a frame into reused (scribbled) host buffers
+    for a in (outs[0]["rgb"], outs[0]["ids"]):
+        a.fill(3)
+    again = ds.render(p, tprim=False, out=outs[0])
+    assert again["rgb"] is outs[0]["rgb"] and (again["rgb"] == ref["rgb"]).all() and (again["ids"] == ref["ids"]).all()
     outs, st = rt.render_batch([])
     assert outs == [] and st.rays_total == 0

@@ -185,3 +185,22 @@ def test_png_writer_round_trip(tmp_path):
             assert (rows[:, 0] == 0).all()
             got = rows[:, 1:].reshape(h, w, 3)
             assert (got == (img[::-1] if bottom_up else img)).all()
+
+
+def test_reused_output_buffers_are_checked():
+    """render_batch / DeviceScene.render with out=: buffers of the wrong shape,
+    type or count are refused before anything is launched."""
+    p = rt.params(64, 32)
+
+    class _NoScene:
+        h = None
+
+    good = {"rgb": np.empty((32, 64, 3), np.uint8), "ids": np.empty((32, 64, 2), np.int32)}
+    with pytest.raises(ValueError):
+        rt.render_batch([(_NoScene(), p)], out=[])
+    with pytest.raises(ValueError):
+        rt.render_batch([(_NoScene(), p)], ids=False, out=[{"rgb": np.empty((32, 64, 4), np.uint8), "ids": None}])
+    with pytest.raises(ValueError):
+        rt.render_batch([(_NoScene(), p)], out=[{"rgb": good["rgb"], "ids": np.empty((32, 64, 2), np.int64)}])
+    with pytest.raises(ValueError):
+        rt.render_batch([(_NoScene(), p)], ids=False, out=[{"rgb": np.empty((32, 128, 3), np.uint8)[:, ::2], "ids": None}])
