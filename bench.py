@@ -191,6 +191,7 @@ def bench_serve(args) -> dict:
         "config": {"workload": f"Poisson arrivals at {args.rates} req/s, {args.duration:g} s each, requests 64x64.."
                                "1024x768 over 8 resident scenes, RGB to host", "slots": args.slots,
                    "capacity_rates": args.capacity_rates,
+                   "host_buffers": "result arrays written before each run starts (pooled response buffers)",
                    "warmup": "one untimed run of each mode at the first rate (other arrivals) before the sweep"},
         "rates": rows,
         "capacity": cap_rows,

@@ -321,13 +321,17 @@ def gen_requests(n: int, rate: float = 0.0, seed: int = 3):
 def serve(items, mode: int = _capi.RTG_SERVE_BATCHED, slots: int = 0, max_batch: int = 0, hbm_budget: int = 0,
           graph_after: int = 0):
     """rtg_serve over [(DeviceScene, rtg_params, arrival_s), ...].
-    Returns ([rgb per request], [(done_s, batch) per request], rtg_serve_stats)."""
+    Returns ([rgb per request], [(done_s, batch) per request], rtg_serve_stats).
+    The result arrays are written once before the loop starts, as a server's
+    pooled response buffers would be: the loop's latencies then hold no
+    first-touch page faults of fresh host memory."""
     lib = _capi.load()
     n = len(items)
     reqs = (_capi.rtg_serve_request * max(1, n))()
     outs = []
     for k, (ds, p, t) in enumerate(items):
         rgb = np.empty((p.height, p.width, 3), np.uint8)
+        rgb.fill(0)
         reqs[k].scene = ds.h
         reqs[k].params = p
         reqs[k].rgb = rgb.ctypes.data_as(C.POINTER(C.c_uint8))

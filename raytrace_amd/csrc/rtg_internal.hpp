@@ -204,6 +204,11 @@ int launch_megakernel(const SceneRes &S, const rtg_params &p, int W, int H, cons
 // Grow `ws` to what one W x H frame of the given depth, fanout and lights
 // reserves before it launches (nothing is allocated while it renders).
 int reserve_frame(Workspace &ws, int W, int H, int spp, int max_level, int fanout, int n_on, Arena &A);
+// Grow `ws` to what a batch of up to n_reqs requests over n_scenes scenes,
+// `samples` level-0 samples and `table_floats` ray-generation table entries
+// reserves before it launches.
+int reserve_batch(Workspace &ws, size_t samples, int n_reqs, int n_scenes, size_t table_floats, int max_level,
+                  int fanout, int n_on, hipStream_t stream, Arena &A);
 int launch_wavefront(const SceneRes &S, const rtg_params &p, int W, int H, const RenderOut &o, hipStream_t stream,
                      Workspace &ws, Arena &A, const int *cancel_host);
 

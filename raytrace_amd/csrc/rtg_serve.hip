@@ -149,28 +149,65 @@ int serve_run(rtg_serve_request *reqs, int n, const rtg_serve_opts &o, rtg_serve
         info[i].n_on = __builtin_popcount(q.scene->res.lights_on);
     }
     while ((int)A.serve_slots.size() < nslots) A.serve_slots.emplace_back(new ServeSlot());
-    // Size every slot for the largest single request before the clock starts,
-    // so that no request waits for a buffer to grow (a reallocation waits for
-    // the whole device).
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return reqs[a].arrival_s < reqs[b].arrival_s; });
+    // Size every slot, before the clock starts, for the largest single
+    // request and the largest batch the loop can form, so that no request
+    // waits for a buffer to grow (a reallocation waits for the whole device).
+    // A batch is a run of at most max_batch consecutive requests in arrival
+    // order: the largest such window bounds its samples, RGB bytes and table
+    // entries, and its queue memory stays within the budget.
     int maxW = 1, maxH = 1, max_spp = 1, max_lvl = 0, max_fan = 0, max_on = 0;
     size_t max_rgb = 0;
+    auto rgb_bytes = [&](int i) { return ((size_t)info[i].W * info[i].H * 3 + 255) & ~size_t(255); };
     for (int i = 0; i < n; ++i) {
         maxW = std::max(maxW, info[i].W), maxH = std::max(maxH, info[i].H);
         max_spp = std::max(max_spp, reqs[i].params.spp > 0 ? (int)reqs[i].params.spp : 1);
         max_lvl = std::max(max_lvl, (int)reqs[i].params.max_level);
         max_fan = std::max(max_fan, info[i].fanout), max_on = std::max(max_on, info[i].n_on);
-        max_rgb = std::max(max_rgb, ((size_t)info[i].W * info[i].H * 3 + 255) & ~size_t(255));
+        max_rgb = std::max(max_rgb, rgb_bytes(i));
+    }
+    size_t win_samples = 0, win_rgb = max_rgb, win_tab = 0;
+    int n_scenes = 0;
+    if (max_batch > 1) {
+        size_t sum_s = 0, sum_r = 0, sum_t = 0;
+        for (int j = 0; j < n; ++j) {
+            const int i = order[j];
+            sum_s += info[i].samples, sum_r += rgb_bytes(i), sum_t += (size_t)info[i].W + info[i].H;
+            if (j >= max_batch) {
+                const int o = order[j - max_batch];
+                sum_s -= info[o].samples, sum_r -= rgb_bytes(o), sum_t -= (size_t)info[o].W + info[o].H;
+            }
+            win_samples = std::max(win_samples, sum_s), win_rgb = std::max(win_rgb, sum_r);
+            win_tab = std::max(win_tab, sum_t);
+        }
+        // the largest level-0 length whose queues fit the budget
+        size_t lo = 0, hi = win_samples;
+        while (lo < hi) {
+            const size_t mid = lo + (hi - lo + 1) / 2;
+            const size_t need = batch_queue_bytes(batch_bucket(mid), max_lvl, max_fan, max_on, A.no_sync_budget);
+            if (need != 0 && need <= budget)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        win_samples = lo;
+        std::vector<const rtg_scene_s *> seen;
+        for (int i = 0; i < n; ++i)
+            if (std::find(seen.begin(), seen.end(), reqs[i].scene) == seen.end()) seen.push_back(reqs[i].scene);
+        n_scenes = (int)std::min<size_t>(seen.size(), (size_t)max_batch);
     }
     for (int k = 0; k < nslots; ++k) {
         ServeSlot &s = *A.serve_slots[k];
         int st = slot_init(s);
-        if (st == RTG_OK && n > 0) st = slot_reserve(s, max_rgb);
+        if (st == RTG_OK && n > 0) st = slot_reserve(s, win_rgb);
         if (st == RTG_OK && n > 0) st = reserve_frame(s.ws, maxW, maxH, max_spp, max_lvl, max_fan, max_on, A);
+        if (st == RTG_OK && win_samples > 0)
+            st = reserve_batch(s.ws, win_samples, std::min(max_batch, n), n_scenes, win_tab, max_lvl, max_fan, max_on,
+                               s.stream, A);
         if (st != RTG_OK) return st;
     }
-    std::vector<int> order(n);
-    for (int i = 0; i < n; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return reqs[a].arrival_s < reqs[b].arrival_s; });
 
     std::vector<int> waiting;  // arrived, not yet admitted (arrival order)
     size_t head = 0;           // first waiting entry not yet admitted

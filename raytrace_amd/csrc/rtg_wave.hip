@@ -825,6 +825,33 @@ static int staging(Workspace &ws, size_t bytes, hipStream_t stream) {
     return RTG_OK;
 }
 
+// Pinned bytes a batch stages: ray-generation tables, request table, scene table.
+static size_t batch_staging_bytes(size_t table_floats, size_t n_reqs, size_t n_scenes, size_t *off_reqs,
+                                  size_t *off_scenes) {
+    *off_reqs = (table_floats * sizeof(float) + 255) & ~size_t(255);
+    *off_scenes = (*off_reqs + n_reqs * sizeof(WaveReq) + 255) & ~size_t(255);
+    return *off_scenes + n_scenes * sizeof(SceneView);
+}
+
+int reserve_batch(Workspace &ws, size_t samples, int n_reqs, int n_scenes, size_t table_floats, int max_level,
+                  int fanout, int n_on, hipStream_t stream, Arena &A) {
+    int st = host_buffers(ws);
+    if (st == RTG_OK) st = ws.side_init();
+    if (st != RTG_OK) return st;
+    if (samples == 0 || n_reqs <= 0) return RTG_OK;
+    if (samples > (size_t)INT32_MAX / 2 || n_reqs > kMaxBatch) return fail(RTG_E_LIMIT, "batch bound too large");
+    const size_t padded = batch_bucket(samples);
+    int n_pad = 1;
+    while (n_pad < n_reqs) n_pad <<= 1;
+    const WavePlan pl = make_plan((int)padded, max_level, fanout, n_on, n_pad, A.no_sync_budget);
+    if (pl.no_sync && (st = reserve_all(ws, pl, A.cus > 0 ? A.cus : 256)) != RTG_OK) return st;
+    if (ws.grow(ws.b_reqs, (size_t)n_pad * sizeof(WaveReq)) || ws.grow(ws.b_scenes, (size_t)n_scenes * sizeof(SceneView)) ||
+        ws.grow(ws.b_chunks, (padded >> 6) * sizeof(int)) || ws.grow(ws.tables, table_floats * sizeof(float)))
+        return fail(RTG_E_NOMEM, "batch tables");
+    size_t off_reqs, off_scenes;
+    return staging(ws, batch_staging_bytes(table_floats, n_pad, n_scenes, &off_reqs, &off_scenes), stream);
+}
+
 int launch_batch(const BatchItem *items, int n, unsigned long long *counts, unsigned long long *retests,
                  hipStream_t stream, Workspace &ws, Arena &A, int capture_at, int *graph_replayed) {
     if (graph_replayed) *graph_replayed = 0;
@@ -893,9 +920,8 @@ int launch_batch(const BatchItem *items, int n, unsigned long long *counts, unsi
         return fail(RTG_E_NOMEM, "batch tables");
     ws.tables_key = 0;  // the single-frame table cache no longer holds
     // stage: ray-generation tables, request table, scene table
-    const size_t off_reqs = (table_floats * sizeof(float) + 255) & ~size_t(255);
-    const size_t off_scenes = (off_reqs + reqs.size() * sizeof(WaveReq) + 255) & ~size_t(255);
-    const size_t bytes = off_scenes + scenes.size() * sizeof(SceneView);
+    size_t off_reqs, off_scenes;
+    const size_t bytes = batch_staging_bytes(table_floats, reqs.size(), scenes.size(), &off_reqs, &off_scenes);
     if ((st = staging(ws, bytes, stream)) != RTG_OK) return st;
     RTG_HIP(hipStreamSynchronize(stream));  // the staging buffer may still feed the previous batch's copies
     uint8_t *stg = static_cast<uint8_t *>(ws.staging);
