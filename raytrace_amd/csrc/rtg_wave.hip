@@ -654,8 +654,9 @@ static int enqueue_wave(const SceneView &S0, const WaveArgs &a, const WavePlan &
         if (ws.grow(ws.spill2, spill_ints(g.occlude) * sizeof(int))) return fail(RTG_E_NOMEM, "spill");
     }
     hipStream_t so = ov ? ws.side : stream;
+    bool forked = false;  // the side stream took work (the join waits for it)
     auto join = [&]() -> int {
-        if (!ov) return RTG_OK;
+        if (!forked) return RTG_OK;
         RTG_HIP(hipEventRecord(ws.ev_join, so));
         RTG_HIP(hipStreamWaitEvent(stream, ws.ev_join, 0));
         return RTG_OK;
@@ -697,17 +698,23 @@ static int enqueue_wave(const SceneView &S0, const WaveArgs &a, const WavePlan &
             S0, args, k, rays, n_ptr, ws.w_hits[k].as<WHit>(), ws.w_cs[k].as<int4>(), ws.w_nodes[k].as<WNode>(),
             ws.w_masks[k].as<unsigned>(), next, ctr + k, ws.w_shadow[k].as<unsigned>());
         ktime_end(A, stream, kt, n_ptr);
-        if (ov) {
+        // The last level has no next level to overlap: its shadow rays stay
+        // on the render stream (with that stream's spill area), where they
+        // run beside the previous level's shadow rays on the side stream.
+        const bool side_k = ov && k < pl.max_level;
+        hipStream_t sk = side_k ? so : stream;
+        if (side_k) {
             RTG_HIP(hipEventRecord(ws.ev_fork[k], stream));
             RTG_HIP(hipStreamWaitEvent(so, ws.ev_fork[k], 0));
+            forked = true;
         }
-        k_fill_slots<<<256, 256, 0, so>>>(ws.w_slots[k].as<float4>(), n_ptr, k == 0 ? pl.n_samples : -1);
+        k_fill_slots<<<256, 256, 0, sk>>>(ws.w_slots[k].as<float4>(), n_ptr, k == 0 ? pl.n_samples : -1);
         if (pl.n_on > 0) {
-            kt = ktime_begin(A, so, KT_OCCLUDE, 0);
-            k_occlude<B><<<grid(g.occlude, n_k * pl.n_on), kBlockThreads, 0, so>>>(
-                S0, oargs, ws.w_nodes[k].as<WNode>(), ws.w_shadow[k].as<unsigned>(), ctr + k,
+            kt = ktime_begin(A, sk, KT_OCCLUDE, 0);
+            k_occlude<B><<<grid(g.occlude, n_k * pl.n_on), kBlockThreads, 0, sk>>>(
+                S0, side_k ? oargs : args, ws.w_nodes[k].as<WNode>(), ws.w_shadow[k].as<unsigned>(), ctr + k,
                 ws.w_masks[k].as<unsigned>());
-            ktime_end(A, so, kt, &ctr[k].n_shadow.v);
+            ktime_end(A, sk, kt, &ctr[k].n_shadow.v);
         }
         RTG_HIP(hipGetLastError());
         last = k;
