@@ -89,10 +89,12 @@ def bench_batch(args) -> dict:
             rt.render_batch(items, ids=False, out=outs)
         t0 = time.perf_counter()
         replays = 0
+        dev_ms = 0.0
         for _ in range(args.steps):
             _, st = rt.render_batch(items, ids=False, out=outs)
             replays += st.graph_replays
-        return (time.perf_counter() - t0) / args.steps, st, replays
+            dev_ms += st.render_ms
+        return (time.perf_counter() - t0) / args.steps, st, replays, dev_ms / args.steps
 
     default_tuning = rt.get_tuning()
     tune = None
@@ -102,12 +104,12 @@ def bench_batch(args) -> dict:
         best, trials = rt.autotune(items, reps=3)
         tune_s = time.perf_counter() - t0
         rt.set_tuning(**default_tuning)
-        default_s, _, _ = timed_batches()
+        default_s, _, _, _ = timed_batches()
         rt.set_tuning(**best)
         tune = {"best": best, "default": default_tuning, "seconds": round(tune_s, 2),
                 "trials": [{**t, "device_ms": round(ms, 3)} for t, ms in trials],
                 "default_ms_per_batch": round(default_s * 1e3, 3)}
-    batch_s, st, replays = timed_batches()
+    batch_s, st, replays, dev_ms = timed_batches()
     rays = st.rays_total
     pouts = [d.render(p, ids=False, tprim=False) for d, p in items]  # warm, and the reused host buffers
     t0 = time.perf_counter()
@@ -126,6 +128,7 @@ def bench_batch(args) -> dict:
                    "rays_per_batch": rays,
                    "pixels_per_batch": sum(p.width * p.height for _, p in items),
                    "scene_triangles": [int(h.info.n_tris) for h in hs], "graph_replayed_steps": replays,
+                   "device_ms_per_batch": round(dev_ms, 3),
                    "tuning": rt.get_tuning()},
         "autotune": tune,
         "plain": {"ms_per_batch": round(plain_s * 1e3, 3), "mrays_per_s": round(rays / plain_s / 1e6, 2),

@@ -4,6 +4,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <thread>
 
 #include "rtg_internal.hpp"
 
@@ -304,6 +305,9 @@ int rtg_shutdown(void) {
         A.out_t.release();
         (void)hipHostFree(A.cancel_host);
         if (A.kitems_host) (void)hipHostFree(A.kitems_host);
+        if (A.pinned_out) (void)hipHostFree(A.pinned_out);
+        A.pinned_out = nullptr;
+        A.pinned_out_bytes = 0;
         for (const Arena::KernelRec &r : A.krecs) A.kevents.push_back(r.e0), A.kevents.push_back(r.e1);
         for (hipEvent_t e : A.kevents) (void)hipEventDestroy(e);
         A.krecs.clear();
@@ -424,6 +428,56 @@ int rtg_scene_build_ms(rtg_scene_t scene, double *ms) {
     return RTG_OK;
 }
 
+// Grow the arena's pinned result staging to `bytes` (the caller holds A.lock;
+// no copy into it is in flight: every batch synchronises before it returns).
+static int ensure_pinned_out(Arena &A, size_t bytes) {
+    if (A.pinned_out_bytes >= bytes) return RTG_OK;
+    if (A.pinned_out) (void)hipHostFree(A.pinned_out);
+    A.pinned_out = nullptr;
+    A.pinned_out_bytes = 0;
+    const size_t n = bytes + bytes / 4;
+    RTG_HIP(hipHostMalloc(reinterpret_cast<void **>(&A.pinned_out), n, hipHostMallocDefault));
+    A.pinned_out_bytes = n;
+    return RTG_OK;
+}
+
+// Run copy(i) for i in [0, n) on up to 8 host threads (the calling thread
+// included); one thread if no more can be started.
+extern "C++" template <class F>
+static void parallel_copies(int n, F &&copy) {
+    std::atomic<int> next{0};
+    auto work = [&]() {
+        for (int i = next++; i < n; i = next++) copy(i);
+    };
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nt = std::max(1, std::min({8, n, (int)hw}));
+    std::vector<std::thread> pool;
+    try {
+        for (int k = 1; k < nt; ++k) pool.emplace_back(work);
+    } catch (...) {
+        // fewer threads: the ones started and this one finish the work
+    }
+    work();
+    for (std::thread &t : pool) t.join();
+}
+
+// Host copies of device results staged in pinned memory, cut into pieces of at
+// most 1 MB so that one large array also spreads over the copy threads.
+struct HostCopy {
+    void *dst;
+    const uint8_t *src;
+    size_t bytes;
+};
+static void copy_out(const HostCopy *cp, int n) {
+    constexpr size_t kPiece = size_t(1) << 20;
+    std::vector<HostCopy> pieces;
+    for (int i = 0; i < n; ++i)
+        for (size_t off = 0; off < cp[i].bytes; off += kPiece)
+            pieces.push_back(HostCopy{static_cast<uint8_t *>(cp[i].dst) + off, cp[i].src + off,
+                                      std::min(kPiece, cp[i].bytes - off)});
+    parallel_copies((int)pieces.size(), [&](int k) { std::memcpy(pieces[k].dst, pieces[k].src, pieces[k].bytes); });
+}
+
 static int render_host(rtg_scene_t scene, const rtg_params *params, uint8_t *rgb, int32_t *ids, float *tprim,
                        rtg_stats *stats, rtg_job_s *job) {
     if (scene == nullptr || rgb == nullptr) return fail(RTG_E_INVALID, "NULL argument");
@@ -447,14 +501,27 @@ static int render_host(rtg_scene_t scene, const rtg_params *params, uint8_t *rgb
     RTG_HIP(hipEventRecord(A.ev0, A.stream));
     st = render_device(scene->res, *params, W, H, o, A.stream, A);
     RTG_HIP(hipEventRecord(A.ev1, A.stream));
+    // Results through the arena's pinned staging: one device-to-host copy per
+    // array, then host copies into the caller's (pageable) arrays.
     unsigned long long cnt[5] = {0, 0, 0, 0, 0};
-    RTG_HIP(hipMemcpyAsync(rgb, o.rgb, npix * 3, hipMemcpyDeviceToHost, A.stream));
-    if (ids) RTG_HIP(hipMemcpyAsync(ids, o.ids, npix * 8, hipMemcpyDeviceToHost, A.stream));
-    if (tprim) RTG_HIP(hipMemcpyAsync(tprim, o.tprim, npix * 4, hipMemcpyDeviceToHost, A.stream));
+    const size_t b_rgb = npix * 3, b_ids = ids ? npix * 8 : 0, b_t = tprim ? npix * 4 : 0;
+    const size_t o_ids = (b_rgb + 255) & ~size_t(255), o_t = (o_ids + b_ids + 255) & ~size_t(255);
+    const int pst = ensure_pinned_out(A, o_t + b_t);
+    if (pst == RTG_OK) {
+        RTG_HIP(hipMemcpyAsync(A.pinned_out, o.rgb, b_rgb, hipMemcpyDeviceToHost, A.stream));
+        if (ids) RTG_HIP(hipMemcpyAsync(A.pinned_out + o_ids, o.ids, b_ids, hipMemcpyDeviceToHost, A.stream));
+        if (tprim) RTG_HIP(hipMemcpyAsync(A.pinned_out + o_t, o.tprim, b_t, hipMemcpyDeviceToHost, A.stream));
+    }
     RTG_HIP(hipMemcpyAsync(cnt, o.counts, sizeof(cnt), hipMemcpyDeviceToHost, A.stream));  // counts + re-tests
     RTG_HIP(hipStreamSynchronize(A.stream));
+    if (pst == RTG_OK) {
+        const HostCopy cp[3] = {{rgb, A.pinned_out, b_rgb}, {ids, A.pinned_out + o_ids, b_ids},
+                                {tprim, A.pinned_out + o_t, b_t}};
+        copy_out(cp, 3);
+    }
     if (job) job->running = 0;
     if (st != RTG_OK) return st;
+    if (pst != RTG_OK) return pst;
     float ms = 0;
     RTG_HIP(hipEventElapsedTime(&ms, A.ev0, A.ev1));
     const bool cancelled = *A.cancel_host != 0;
@@ -614,17 +681,26 @@ int rtg_render_batch(const rtg_request *reqs, int n, rtg_stats *stats) {
     if (st == RTG_OK) st = launch_batch(items.data(), n, counts, counts + 4, s, A.ws, A, eager ? 0 : 1, &replayed);
     if (st == RTG_OK) st = A.ws.mark_done(s);
     RTG_HIP(hipEventRecord(A.ev1, s));
+    // Results: one copy of the whole output region into pinned memory, then a
+    // scatter into the callers' (pageable) arrays on host threads, instead of
+    // one staged pageable copy per request.
     unsigned long long cnt[5] = {0, 0, 0, 0, 0};
+    if (st == RTG_OK) st = ensure_pinned_out(A, rgb_bytes + ids_bytes);
     if (st == RTG_OK) {
-        for (int i = 0; i < n; ++i) {
-            const size_t npix = (size_t)items[i].W * items[i].H;
-            RTG_HIP(hipMemcpyAsync(reqs[i].rgb, items[i].rgb, npix * 3, hipMemcpyDeviceToHost, s));
-            if (reqs[i].ids) RTG_HIP(hipMemcpyAsync(reqs[i].ids, items[i].ids, npix * 8, hipMemcpyDeviceToHost, s));
-        }
+        RTG_HIP(hipMemcpyAsync(A.pinned_out, A.out_rgb.p, rgb_bytes, hipMemcpyDeviceToHost, s));
+        if (ids_bytes)
+            RTG_HIP(hipMemcpyAsync(A.pinned_out + rgb_bytes, A.out_ids.p, ids_bytes, hipMemcpyDeviceToHost, s));
         RTG_HIP(hipMemcpyAsync(cnt, counts, sizeof(cnt), hipMemcpyDeviceToHost, s));
     }
     RTG_HIP(hipStreamSynchronize(s));
     if (st != RTG_OK) return st;
+    std::vector<HostCopy> cp;
+    for (int i = 0; i < n; ++i) {
+        const size_t npix = (size_t)items[i].W * items[i].H;
+        cp.push_back(HostCopy{reqs[i].rgb, A.pinned_out + off_rgb[i], npix * 3});
+        if (reqs[i].ids) cp.push_back(HostCopy{reqs[i].ids, A.pinned_out + rgb_bytes + off_ids[i], npix * 8});
+    }
+    copy_out(cp.data(), (int)cp.size());
     float ms = 0;
     RTG_HIP(hipEventElapsedTime(&ms, A.ev0, A.ev1));
     if (stats) {

@@ -140,6 +140,10 @@ struct Arena {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBuf counters, out_rgb, out_ids, out_t;
+    // Pinned staging of a batch's results: one device-to-host copy per batch,
+    // then a threaded scatter into the callers' arrays.
+    uint8_t *pinned_out = nullptr;
+    size_t pinned_out_bytes = 0;
     Workspace ws;
     std::vector<std::unique_ptr<ServeSlot>> serve_slots;
     int *cancel_host = nullptr;  // pinned, mapped
