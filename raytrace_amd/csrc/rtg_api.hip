@@ -461,14 +461,10 @@ static void parallel_copies(int n, F &&copy) {
     for (std::thread &t : pool) t.join();
 }
 
-// Host copies of device results staged in pinned memory, cut into pieces of at
-// most 1 MB so that one large array also spreads over the copy threads.
-struct HostCopy {
-    void *dst;
-    const uint8_t *src;
-    size_t bytes;
-};
-static void copy_out(const HostCopy *cp, int n) {
+}  // extern "C"
+
+namespace rtg {
+void copy_out(const HostCopy *cp, int n) {
     constexpr size_t kPiece = size_t(1) << 20;
     std::vector<HostCopy> pieces;
     for (int i = 0; i < n; ++i)
@@ -477,6 +473,9 @@ static void copy_out(const HostCopy *cp, int n) {
                                       std::min(kPiece, cp[i].bytes - off)});
     parallel_copies((int)pieces.size(), [&](int k) { std::memcpy(pieces[k].dst, pieces[k].src, pieces[k].bytes); });
 }
+}  // namespace rtg
+
+extern "C" {
 
 static int render_host(rtg_scene_t scene, const rtg_params *params, uint8_t *rgb, int32_t *ids, float *tprim,
                        rtg_stats *stats, rtg_job_s *job) {

@@ -205,6 +205,15 @@ int launch_megakernel(const SceneRes &S, const rtg_params &p, int W, int H, cons
                       Arena &A);
 // Launch one frame of the wavefront engine on `stream` (asynchronous) with
 // the buffers of `ws`; `cancel_host` (optional) is polled between levels.
+// Host copies of device results staged in pinned memory, run on up to 8
+// host threads in pieces of at most 1 MB.
+struct HostCopy {
+    void *dst;
+    const uint8_t *src;
+    size_t bytes;
+};
+void copy_out(const HostCopy *cp, int n);
+
 // Grow `ws` to what one W x H frame of the given depth, fanout and lights
 // reserves before it launches (nothing is allocated while it renders).
 int reserve_frame(Workspace &ws, int W, int H, int spp, int max_level, int fanout, int n_on, Arena &A);

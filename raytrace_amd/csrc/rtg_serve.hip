@@ -228,10 +228,11 @@ int serve_run(rtg_serve_request *reqs, int n, const rtg_serve_opts &o, rtg_serve
                 st = fail(RTG_E_DEVICE, std::string("serve batch: ") + hipGetErrorString(e));
                 break;
             }
-            for (size_t j = 0; j < s.reqs.size(); ++j) {
-                rtg_serve_request &q = reqs[s.reqs[j]];
-                std::memcpy(q.rgb, s.pinned + s.offs[j], (size_t)info[s.reqs[j]].W * info[s.reqs[j]].H * 3);
-            }
+            std::vector<HostCopy> cp(s.reqs.size());
+            for (size_t j = 0; j < s.reqs.size(); ++j)
+                cp[j] = HostCopy{reqs[s.reqs[j]].rgb, s.pinned + s.offs[j],
+                                 (size_t)info[s.reqs[j]].W * info[s.reqs[j]].H * 3};
+            copy_out(cp.data(), (int)cp.size());
             const double t = since(t0);
             for (int id : s.reqs) {
                 reqs[id].done_s = t;
