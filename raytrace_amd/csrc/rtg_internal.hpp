@@ -87,7 +87,8 @@ struct Workspace {
     int order_before(hipStream_t s);  // make `s` wait for the previous render
     int mark_done(hipStream_t s);     // record `done` on `s`
     // Side stream of the level loop: the shadow rays of level k run on it
-    // while the render stream traces level k + 1 (own spill area: two
+    // while the render stream traces level k + 1, and the last level's
+    // shadow rays run on the render stream beside it (own spill area: two
     // traversal kernels at once).  Created on first use, default priority.
     hipStream_t side = nullptr;
     hipEvent_t ev_fork[kLevels] = {}, ev_join = nullptr;

@@ -646,7 +646,8 @@ static int enqueue_wave(const SceneView &S0, const WaveArgs &a, const WavePlan &
     // The shadow rays of level k (k_fill_slots, k_occlude) depend only on
     // k_shade(k), and level k + 1 does not depend on them: with `overlap`
     // they run on the workspace's side stream, concurrently with the next
-    // level, and join the render stream before k_combine.
+    // level, and join the render stream before k_combine (the last level's
+    // stay on the render stream, see below).
     const bool ov = overlap && pl.n_on > 0;
     if (ov) {
         int st = ws.side_init();
