@@ -961,7 +961,7 @@ int launch_batch(const BatchItem *items, int n, unsigned long long *counts, unsi
                          ((uint64_t)fanout << 50) ^ ((uint64_t)n_on << 53) ^ ((uint64_t)any_retest << 60) ^
                          (ws.epoch * 0x9E3779B97F4A7C15ull) ^ ((uint64_t)(uintptr_t)counts * 0xC2B2AE3D27D4EB4Full) ^
                          ((uint64_t)(uintptr_t)retests * 0x165667B19E3779F9ull) ^
-                         (A.tune.key() * 0x27D4EB2F165667C5ull);
+                         (A.tune.key() * 0x27D4EB2F165667C5ull) ^ ((uint64_t)one_stream << 63);
     // each level's shadow rays beside the next level on the side stream
     // (`overlap`), or all on `stream`
     auto enqueue = [&](bool overlap) -> int {
