@@ -387,6 +387,22 @@ def main() -> int:
     kt = rt.kernel_times()
     dom = max(kt, key=lambda k: kt[k][2])
     launches, items, ms = kt[dom]
+    # The same again with every kernel on the render stream: the dominant
+    # kernel's launch time when it has the GPU to itself.
+    alone_ms = None
+    if eng_name == "wave" and not args.one_stream:
+        pa = rt.params(W, H, max_level=DEPTH, engine=engine,
+                       flags=rt.RTG_FLAG_KERNEL_TIMES | _capi.RTG_FLAG_ONE_STREAM)
+        rt.kernel_times()
+        for _ in range(args.steps):
+            rc = lib.rtg_render_device(ds.h, C.byref(pa), C.c_void_p(rgb.data_ptr()), None, None,
+                                       C.c_void_p(counts.data_ptr()), sp)
+            if rc != 0:
+                raise RuntimeError(f"rtg_render_device failed: {_capi.last_error()}")
+        stream.synchronize()
+        ka = rt.kernel_times()
+        if dom in ka:
+            alone_ms = ka[dom][2] / ka[dom][0]
 
     # algorithmic traffic of the frame's ray set, canonical oracle BVH, tile sample
     from oracle import pyoracle as po
@@ -415,6 +431,10 @@ def main() -> int:
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(eng_name, dom),
                 "kernel": dom, "launches_per_frame": launches // args.steps, "avg_launch_ms": round(avg_ms, 4),
                 "work_unit": unit, "units_per_launch": round(items / launches), "bytes_per_unit": round(per_unit, 1),
+                "alone": None if alone_ms is None else {
+                    "avg_launch_ms": round(alone_ms, 4),
+                    "frac": round(bytes_per_launch / (alone_ms / 1000.0) / 1e9 / HBM_PEAK_GBS, 4),
+                    "what": "the same frames with every kernel on the render stream (RTG_FLAG_ONE_STREAM)"},
                 "frame": {"device_ms": round(frame_ms, 4),
                           "achieved_GBs": round(rays_per_frame * tm["per_reference_ray"] / frame_ms / 1e6, 1),
                           "bytes_per_reference_ray": round(tm["per_reference_ray"], 1),
