@@ -18,13 +18,14 @@ import ctypes as C
 import numpy as np
 
 from . import _capi
-from ._capi import (RTG_ENGINE_AUTO, RTG_ENGINE_MEGAKERNEL, RTG_ENGINE_WAVEFRONT, RTG_FLAG_FP64_RETEST,
-                    RTG_FLAG_KERNEL_TIMES,
+from ._capi import (RTG_ENGINE_AUTO, RTG_ENGINE_MEGAKERNEL, RTG_ENGINE_WAVEFRONT, RTG_FLAG_EAGER,
+                    RTG_FLAG_FP64_RETEST, RTG_FLAG_KERNEL_TIMES, RTG_FLAG_ONE_STREAM,
                     RTG_MODE_RAYTRACE, RtgError, check, rtg_gen_info, rtg_gen_opts, rtg_params, rtg_stats)
 
 __all__ = [
     "init", "shutdown", "params", "HostScene", "DeviceScene", "RtgError", "RTG_ENGINE_AUTO",
-    "RTG_ENGINE_MEGAKERNEL", "RTG_ENGINE_WAVEFRONT", "RTG_FLAG_FP64_RETEST", "RTG_MODE_RAYTRACE",
+    "RTG_ENGINE_MEGAKERNEL", "RTG_ENGINE_WAVEFRONT", "RTG_FLAG_EAGER", "RTG_FLAG_FP64_RETEST",
+    "RTG_FLAG_KERNEL_TIMES", "RTG_FLAG_ONE_STREAM", "RTG_MODE_RAYTRACE",
 ]
 
 
