@@ -1,0 +1,52 @@
+"""bench.py on the CPU: the one-GPU contract and the algorithmic-traffic model
+that the roofline figure divides by (per traced ray of each kind; shadow rays
+that cannot change the colour are not traced and cost 0)."""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from types import SimpleNamespace
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+import bench  # noqa: E402
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120, env={**os.environ, "HIP_VISIBLE_DEVICES": ""})
+    assert r.returncode != 0
+    assert "one GPU" in (r.stdout + r.stderr)
+
+
+def _traffic(**kw):
+    base = dict(rays=0, node_visits=0, tri_tests=0, sph_tests=0, shadow_rays=0, shadow_node_visits=0,
+                shadow_tri_tests=0, shadow_sph_tests=0, shadow_skipped=0)
+    base.update(kw)
+    return SimpleNamespace(**base)
+
+
+def test_one_shadow_ray_and_one_closest_ray():
+    # one closest-hit ray: 3 nodes, 2 triangles; one traced shadow ray: 5 nodes,
+    # 1 triangle, 1 sphere; one skipped shadow ray
+    tr = _traffic(rays=2, node_visits=8, tri_tests=3, sph_tests=1, shadow_rays=1, shadow_node_visits=5,
+                  shadow_tri_tests=1, shadow_sph_tests=1, shadow_skipped=1)
+    m = bench.traffic_model(tr)
+    closest = 32 + 32 + 3 * 64 + 2 * 48
+    shadow = 32 + 32 + 5 * 64 + 48 + 16
+    assert m["per_closest_ray"] == pytest.approx(closest)
+    assert m["per_shadow_ray"] == pytest.approx(shadow)
+    # three reference-semantic rays, the skipped one at 0 bytes
+    assert m["per_reference_ray"] == pytest.approx((closest + shadow) / 3)
+    assert m["shadow_skipped_frac"] == pytest.approx(0.5)
+
+
+def test_traffic_scales_with_rays():
+    one = bench.traffic_model(_traffic(rays=1, node_visits=10, tri_tests=4))
+    many = bench.traffic_model(_traffic(rays=1000, node_visits=10000, tri_tests=4000))
+    assert one["per_closest_ray"] == pytest.approx(many["per_closest_ray"])
+    assert one["per_shadow_ray"] == 0.0  # no shadow rays: nothing divided by zero
