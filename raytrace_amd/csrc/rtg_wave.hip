@@ -284,7 +284,7 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_shade(SceneView S0, WaveAr
                                                          const unsigned *n_ptr, const WHit *hits, const int4 *cs,
                                                          WNode *nodes,
                                                          unsigned *masks, WRay *next, LevelCtr *ctr,
-                                                         unsigned *shadow) {
+                                                         unsigned *shadow, float4 *slots) {
     const int n = k == 0 ? A.n_samples : (int)*n_ptr;
     WaveQueue wq(ctr->sched[1], n, A.grab_lg);
     uint32_t c_shadow = 0;
@@ -375,6 +375,10 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_shade(SceneView S0, WaveAr
             }
             nodes[i] = nd;
             masks[i] = 0;
+            // the node's two child slots start empty (k_combine of the next
+            // level writes the children it has)
+            slots[2 * i] = make_float4(0.0f, 0.0f, 0.0f, kMiss);
+            slots[2 * i + 1] = make_float4(0.0f, 0.0f, 0.0f, kMiss);
         }
         const int cb = wave_append(&ctr->n_next.v, n_child);
         if (n_child > 0) next[cb] = c0;
@@ -435,12 +439,6 @@ __global__ __launch_bounds__(kBlockThreads, 8) void k_occlude(SceneView S0, Wave
         sr.self_test = self_testable(S, sr.r, sr.src, lt.dis);
         if (trace_any(S, sr, lt.dis, st)) atomicOr(&masks[i], 1u << li);
     }
-}
-
-__global__ void k_fill_slots(float4 *slots, const unsigned *n_ptr, int n_fixed) {
-    const int n = 2 * (n_fixed >= 0 ? n_fixed : (int)*n_ptr);
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        slots[i] = make_float4(0.0f, 0.0f, 0.0f, kMiss);
 }
 
 template <bool B>
@@ -643,7 +641,7 @@ static int enqueue_wave(const SceneView &S0, const WaveArgs &a, const WavePlan &
         if (ws.grow(ws.w_qcount, (size_t)Workspace::kLevels * sizeof(LevelCtr)))
             return fail(RTG_E_NOMEM, "queue counters");
     }
-    // The shadow rays of level k (k_fill_slots, k_occlude) depend only on
+    // The shadow rays of level k (k_occlude) depend only on
     // k_shade(k), and level k + 1 does not depend on them: with `overlap`
     // they run on the workspace's side stream, concurrently with the next
     // level, and join the render stream before k_combine (the last level's
@@ -697,7 +695,7 @@ static int enqueue_wave(const SceneView &S0, const WaveArgs &a, const WavePlan &
         kt = ktime_begin(A, stream, KT_SHADE, n_k);
         k_shade<B><<<grid(g.shade, n_k), kBlockThreads, 0, stream>>>(
             S0, args, k, rays, n_ptr, ws.w_hits[k].as<WHit>(), ws.w_cs[k].as<int4>(), ws.w_nodes[k].as<WNode>(),
-            ws.w_masks[k].as<unsigned>(), next, ctr + k, ws.w_shadow[k].as<unsigned>());
+            ws.w_masks[k].as<unsigned>(), next, ctr + k, ws.w_shadow[k].as<unsigned>(), ws.w_slots[k].as<float4>());
         ktime_end(A, stream, kt, n_ptr);
         // The last level has no next level to overlap: its shadow rays stay
         // on the render stream (with that stream's spill area), where they
@@ -709,7 +707,6 @@ static int enqueue_wave(const SceneView &S0, const WaveArgs &a, const WavePlan &
             RTG_HIP(hipStreamWaitEvent(so, ws.ev_fork[k], 0));
             forked = true;
         }
-        k_fill_slots<<<256, 256, 0, sk>>>(ws.w_slots[k].as<float4>(), n_ptr, k == 0 ? pl.n_samples : -1);
         if (pl.n_on > 0) {
             kt = ktime_begin(A, sk, KT_OCCLUDE, 0);
             k_occlude<B><<<grid(g.occlude, n_k * pl.n_on), kBlockThreads, 0, sk>>>(
