@@ -442,7 +442,7 @@ __global__ __launch_bounds__(kBlockThreads, 8) void k_occlude(SceneView S0, Wave
 }
 
 template <bool B>
-__global__ __launch_bounds__(kBlockThreads) void k_combine(SceneView S0, WaveArgs A, int k, const WRay *rays,
+__global__ __launch_bounds__(kBlockThreads, 8) void k_combine(SceneView S0, WaveArgs A, int k, const WRay *rays,
                                                            const unsigned *n_ptr, const WNode *nodes,
                                                            const unsigned *masks, const float4 *slots,
                                                            float4 *parent_slots, float4 *samples) {
@@ -732,7 +732,7 @@ static int enqueue_wave(const SceneView &S0, const WaveArgs &a, const WavePlan &
     if (cancel_host && *cancel_host) return RTG_OK;
     if (!pl.no_sync && ws.grow(ws.w_samples, (size_t)pl.n_samples * sizeof(float4)))
         return fail(RTG_E_NOMEM, "samples");
-    const int blocks = std::max(8, cus * 8);
+    const int blocks = std::max(8, cus * 16);
     for (int k = last; k >= 0; --k) {
         const WRay *rays = k == 0 ? nullptr : ws.w_rays[k].as<WRay>();
         const unsigned *n_ptr = k == 0 ? nullptr : &ctr[k - 1].n_next.v;
