@@ -195,19 +195,29 @@ __device__ inline int sample_req(const WaveArgs &A, int i) {
         return 0;
 }
 
+// A persistent traversal grid is sized for the level's bound, and the deep
+// levels hold far fewer items: a block whose waves would each start past the
+// last 64-item chunk finds no work, and leaves before copying the top nodes
+// into LDS (block-uniform, so ahead of the barrier there).  The blocks that
+// stay take every chunk: waves keep grabbing until the queue is empty.
+__device__ inline bool surplus_block(int n) {
+    return (int)blockIdx.x * (kBlockThreads / 64) >= ((n + 63) >> 6);
+}
+
 // R: some request asks for the FP64 near-tie re-test (a separate kernel, so
 // the plain closest-hit walk keeps its registers).
 template <bool B, bool R>
 __global__ __launch_bounds__(kBlockThreads, 6) void k_extend(SceneView S0, WaveArgs A, int k, const WRay *rays,
                                                              const unsigned *n_ptr, WHit *hits, int4 *cs,
                                                              LineCtr *sched) {
+    const int n = k == 0 ? A.n_samples : (int)*n_ptr;
+    if (surplus_block(n)) return;
     __shared__ int lds[kLdsStack * kBlockThreads];
     DevStack st = make_stack(lds, A.spill);
     if constexpr (!B) {  // one scene: its top nodes in LDS
         __shared__ BvhNode top_nodes[kTopNodes];
         stack_top_nodes(st, top_nodes, S0);
     }
-    const int n = k == 0 ? A.n_samples : (int)*n_ptr;
     WaveQueue wq(sched, n, A.grab_lg);
     uint32_t c_prim = 0, c_refl = 0, c_refr = 0;
     int n_retest = 0;
@@ -413,13 +423,14 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_shade(SceneView S0, WaveAr
 template <bool B>
 __global__ __launch_bounds__(kBlockThreads, 8) void k_occlude(SceneView S0, WaveArgs A, const WNode *nodes,
                                                            const unsigned *shadow, LevelCtr *ctr, unsigned *masks) {
+    const int n = (int)ctr->n_shadow.v;
+    if (surplus_block(n)) return;
     __shared__ int lds[kLdsStack * kBlockThreads];
     DevStack st = make_stack(lds, A.spill);
     if constexpr (!B) {  // one scene: its top nodes in LDS
         __shared__ BvhNode top_nodes[kTopNodes];
         stack_top_nodes(st, top_nodes, S0);
     }
-    const int n = (int)ctr->n_shadow.v;
     WaveQueue wq(ctr->sched[2], n, A.grab_lg);
     for (int base = wq.next(); base >= 0; base = wq.next()) {
         const int j = base + __lane_id();
