@@ -50,7 +50,7 @@ def traffic_model(tr) -> dict:
     }
 
 
-# Launch tuning of the 1080p frame, as rtg_autotune picks it on MI355X.
+# Launch tuning of the 1080p frame, chosen by rtg_autotune on MI355X.
 FRAME_TUNING = dict(tile_w=8, grab=1, grid_pct=75)
 
 
@@ -354,11 +354,10 @@ def main() -> int:
         if rc != 0:
             raise RuntimeError(f"rtg_render_device failed: {_capi.last_error()}")
 
-    # Launch tuning of the frame path: the one rtg_autotune picks for this
-    # frame on MI355X (persistent grids at 75% of the resident blocks, so the
-    # render and side streams' kernels run side by side: 2.849 against 2.927
-    # ms per frame at the library default of 100%), installed as a caller
-    # that renders this frame repeatedly would; --autotune re-runs the tuner,
+    # Launch tuning of the frame path, chosen by rtg_autotune on this frame on
+    # MI355X (persistent grids at 75% of the resident blocks, so the render
+    # and side streams' kernels run side by side), installed as a caller that
+    # renders this frame repeatedly would; --autotune re-runs the tuner,
     # --default-tuning keeps the library default.
     tune = None
     if args.autotune:
