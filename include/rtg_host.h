@@ -93,6 +93,15 @@ int rtg_host_resize(rtg_host_scene_t scene, int width, int height);
  * `uploads` (optional) receives how many times the scene was uploaded. */
 int rtg_host_raytrace(rtg_host_scene_t scene, int type, int max_level, int frames, uint8_t *out,
                       double *use_time, uint64_t *uploads);
+/* RayTracer::start(type, tnum), then RayTracer::stop() (RayTracer.cpp:698-701)
+ * `stop_after_s` seconds later, then waiting for isFinish, as the viewer's
+ * close handler does (main.cpp:699-705).  The frame lands in `out` (2048*2048*3
+ * bytes): tiles rendered before the stop keep their pixels, the rest stay 127.
+ * Returns the frame's status: RTG_E_CANCELLED when the stop reached it.
+ * `finished_before_stop` (optional) receives 1 when isFinish was already set
+ * at the stop. */
+int rtg_host_raytrace_stop(rtg_host_scene_t scene, int type, int max_level, double stop_after_s, uint8_t *out,
+                           int *finished_before_stop);
 
 /* Write a frame as an 8-bit RGB PNG: what the reference's viewer shows of
  * RayTracer::output (main.cpp:207-208, an RGB texture whose row 0 is the

@@ -1255,3 +1255,90 @@ void oracle_raygen(const rtg_camera *cam, int width, int height, int px, int py,
     o[0] = cam->position.x, o[1] = cam->position.y, o[2] = cam->position.z, o[3] = cam->position.w;
     d[0] = dir.x, d[1] = dir.y, d[2] = dir.z, d[3] = dir.w;
 }
+
+/* Plane::intersect (Basic3DObject.cpp:376-410) of a ray from another object:
+ * the hit distance (1e20: miss) and the texture coordinates. */
+float oracle_plane_hit(const float o[4], const float d[4], const float p0[4], const float n[4], const float ax[4],
+                       const float ay[4], float uv[2]) {
+    RAY32 r;
+    memset(&r, 0, sizeof(r));
+    r.origin = ld4(o), r.direction = ld4(d), r.type = MY_RAY_BASERAY;
+    rtg_object ob;
+    memset(&ob, 0, sizeof(ob));
+    ob.type = RTG_OBJ_PLANE;
+    ob.position = (rtg_vec4){p0[0], p0[1], p0[2], p0[3]};
+    ob.normal = (rtg_vec4){n[0], n[1], n[2], n[3]};
+    ob.axisx = (rtg_vec4){ax[0], ax[1], ax[2], ax[3]};
+    ob.axisy = (rtg_vec4){ay[0], ay[1], ay[2], ay[3]};
+    HIT32 hr, out;
+    memset(&hr, 0, sizeof(hr));
+    hr.obj = NO_OBJ, hr.distance = 1e20f;
+    if (!plane_hit32(&r, &hr, &ob, 0, &out)) return 1e20f;
+    if (uv) uv[0] = out.tu, uv[1] = out.tv;
+    return out.distance;
+}
+
+/* Box::intersect (Basic3DObject.cpp:274-300) of a ray from another object:
+ * the hit distance (1e20: miss) and the face normal. */
+float oracle_box_hit(const float o[4], const float d[4], const float pos[4], const float bmin[4], const float bmax[4],
+                     float normal[4]) {
+    RAY32 r;
+    memset(&r, 0, sizeof(r));
+    r.origin = ld4(o), r.direction = ld4(d), r.type = MY_RAY_BASERAY;
+    rtg_object ob;
+    memset(&ob, 0, sizeof(ob));
+    ob.type = RTG_OBJ_CUBE;
+    ob.position = (rtg_vec4){pos[0], pos[1], pos[2], pos[3]};
+    ob.vmin = (rtg_vec4){bmin[0], bmin[1], bmin[2], bmin[3]};
+    ob.vmax = (rtg_vec4){bmax[0], bmax[1], bmax[2], bmax[3]};
+    HIT32 hr, out;
+    memset(&hr, 0, sizeof(hr));
+    hr.obj = NO_OBJ, hr.distance = 1e20f;
+    if (!box_hit32(&r, &hr, &ob, 0, &out)) return 1e20f;
+    if (normal) normal[0] = out.normal.x, normal[1] = out.normal.y, normal[2] = out.normal.z, normal[3] = out.normal.w;
+    return out.distance;
+}
+
+/* BallPlane::intersect (Basic3DObject.cpp:486-552) on the frame (pos, ax, ay)
+ * with sphere radius r.  The ray leaves sphere `src` (its index in the
+ * reference's loop order, cnt - 1; -1: it comes from elsewhere) with the
+ * given inside flag and type (MY_RAY_*).  Returns the hit distance (1e20:
+ * miss) and, for a hit, out_i = {sphere index, inside flag of the hit};
+ * out_f = {normal x, y, z, rfr}. */
+float oracle_ballplane_hit(const float o[4], const float d[4], const float pos[4], const float ax[4],
+                           const float ay[4], float radius, float mtl_rfr, int src, int inside, int type,
+                           int32_t out_i[2], float out_f[4]) {
+    RAY32 r;
+    memset(&r, 0, sizeof(r));
+    r.origin = ld4(o), r.direction = ld4(d), r.type = type, r.isInside = inside ? 1 : 0;
+    rtg_object ob;
+    memset(&ob, 0, sizeof(ob));
+    ob.type = RTG_OBJ_BALLPLANE;
+    ob.position = (rtg_vec4){pos[0], pos[1], pos[2], pos[3]};
+    ob.axisx = (rtg_vec4){ax[0], ax[1], ax[2], ax[3]};
+    ob.axisy = (rtg_vec4){ay[0], ay[1], ay[2], ay[3]};
+    ob.radius = radius, ob.radius_sqr = radius * radius;
+    rtg_material m;
+    memset(&m, 0, sizeof(m));
+    m.rfr = mtl_rfr;
+    HIT32 hr, out;
+    memset(&hr, 0, sizeof(hr));
+    hr.obj = src >= 0 ? BALLPLANE_ID(0, src + 1) : NO_OBJ, hr.distance = 1e20f;
+    if (!ballplane_hit32(&r, &hr, &ob, 0, &m, &out)) return 1e20f;
+    if (out_i) out_i[0] = out.id_prim, out_i[1] = out.isInside;
+    if (out_f) out_f[0] = out.normal.x, out_f[1] = out.normal.y, out_f[2] = out.normal.z, out_f[3] = out.rfr;
+    return out.distance;
+}
+
+/* Color(const Texture*, const Coord2D&) (3DElement.cpp:430-450): the texel
+ * colour of texture `tex` (NULL: no texture) at (u, v). */
+void oracle_texel(const rtg_texture *tex, float u, float v, float rgb[3]) {
+    rtg_scene_desc d;
+    memset(&d, 0, sizeof(d));
+    d.textures = tex, d.n_textures = tex ? 1 : 0;
+    oscene S;
+    memset(&S, 0, sizeof(S));
+    S.d = &d;
+    V32 c = texel32(&S, tex ? 0 : -1, u, v);
+    rgb[0] = c.x, rgb[1] = c.y, rgb[2] = c.z;
+}

@@ -95,6 +95,14 @@ float oracle_border_test_ex(const float o[4], const float d[4], const float bmin
                             int32_t mask[8]);
 void oracle_raygen(const rtg_camera *cam, int width, int height, int px, int py, float o[4], float d[4]);
 uint8_t oracle_quantize(float c);
+float oracle_plane_hit(const float o[4], const float d[4], const float p0[4], const float n[4], const float ax[4],
+                       const float ay[4], float uv[2]);
+float oracle_box_hit(const float o[4], const float d[4], const float pos[4], const float bmin[4], const float bmax[4],
+                     float normal[4]);
+float oracle_ballplane_hit(const float o[4], const float d[4], const float pos[4], const float ax[4],
+                           const float ay[4], float radius, float mtl_rfr, int src, int inside, int type,
+                           int32_t out_i[2], float out_f[4]);
+void oracle_texel(const rtg_texture *tex, float u, float v, float rgb[3]);
 
 #ifdef __cplusplus
 }

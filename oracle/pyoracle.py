@@ -78,6 +78,15 @@ def load(path: str | None = None) -> C.CDLL:
     lib.oracle_raygen.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, _F4, _F4]
     lib.oracle_quantize.argtypes = [C.c_float]
     lib.oracle_quantize.restype = C.c_uint8
+    lib.oracle_plane_hit.argtypes = [_F4] * 6 + [C.c_float * 2]
+    lib.oracle_plane_hit.restype = C.c_float
+    lib.oracle_box_hit.argtypes = [_F4] * 6
+    lib.oracle_box_hit.restype = C.c_float
+    lib.oracle_ballplane_hit.argtypes = [_F4] * 5 + [C.c_float, C.c_float, C.c_int, C.c_int, C.c_int,
+                                                     C.c_int32 * 2, _F4]
+    lib.oracle_ballplane_hit.restype = C.c_float
+    lib.oracle_texel.argtypes = [P, C.c_float, C.c_float, C.c_float * 3]
+    lib.oracle_texel.restype = None
     _libs[path] = lib
     return lib
 

@@ -145,6 +145,15 @@ class HostScene:
                                           C.byref(t), C.byref(up)), "RayTracer::start")
         return (out, t.value, up.value) if with_uploads else (out, t.value)
 
+    def raytrace_stopped(self, type_: int, max_level: int, stop_after_s: float):
+        """RayTracer::start, RayTracer::stop after `stop_after_s` seconds, then wait
+        for isFinish; returns (2048x2048x3 buffer, status, finished before the stop)."""
+        out = np.full((2048 * 2048 * 3,), 127, np.uint8)
+        fin = C.c_int()
+        rc = self._lib.rtg_host_raytrace_stop(self.h, type_, max_level, float(stop_after_s),
+                                              out.ctypes.data_as(C.c_void_p), C.byref(fin))
+        return out, rc, bool(fin.value)
+
 
 class DeviceScene:
     """A scene resident in HBM (rtg_scene_create)."""

@@ -230,6 +230,7 @@ SIGNATURES = {
     "rtg_host_set_env_light": (_I, [_P, C.POINTER(_F)]),
     "rtg_host_resize": (_I, [_P, _I, _I]),
     "rtg_host_raytrace": (_I, [_P, _I, _I, _I, _P, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "rtg_host_raytrace_stop": (_I, [_P, _I, _I, C.c_double, _P, C.POINTER(C.c_int)]),
 }
 
 _lib = None

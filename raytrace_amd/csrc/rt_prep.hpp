@@ -130,4 +130,39 @@ inline size_t batch_bucket(size_t total) {
     return padded;
 }
 
+// Most rays one level of a wavefront may hold.  A shadow-ray entry packs the
+// index of its hit node with the light index as (node << 5 | light) in 32
+// bits, and entries are appended at 32-bit signed offsets: a level holds at
+// most 2^27 - 1 rays and at most 2^31 - 1 shadow entries (rays x lights).
+inline size_t level_ray_cap(int n_on) {
+    const size_t by_node = (size_t(1) << 27) - 1;
+    const size_t by_entries = ((size_t(1) << 31) - 1) / (size_t)(n_on > 1 ? n_on : 1);
+    return by_node < by_entries ? by_node : by_entries;
+}
+
+// Rows of pixel blocks per pass of a single frame (`per_row` level-0
+// samples each).  Rays that branch (fanout 2: reflection and refraction) run
+// in passes of at most `branch_pass` samples; otherwise level 0 is the
+// largest level and a pass takes as many rows as the level cap allows.  Deeper
+// levels of a branching frame are checked against the cap as they are
+// counted.
+inline int frame_rows_per_pass(int rows, size_t per_row, int fanout, int n_on, size_t branch_pass) {
+    size_t lim = level_ray_cap(n_on);
+    if (fanout >= 2 && branch_pass < lim) lim = branch_pass;
+    const size_t r = per_row > 0 ? lim / per_row : (size_t)rows;
+    return (int)(r < 1 ? 1 : (r > (size_t)rows ? (size_t)rows : r));
+}
+
+// End of the group of consecutive requests that starts at `begin`: the
+// longest run whose padded level-0 length (batch_bucket) stays within `cap`.
+// A request that alone exceeds the cap is a group of one, rendered as a
+// frame in passes.
+inline int batch_group_end(const size_t *samples, int n, int begin, size_t cap) {
+    size_t acc = samples[begin];
+    int i = begin + 1;
+    if (batch_bucket(acc) > cap) return i;
+    while (i < n && batch_bucket(acc + samples[i]) <= cap) acc += samples[i++];
+    return i;
+}
+
 }  // namespace rtg

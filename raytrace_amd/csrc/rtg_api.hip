@@ -6,6 +6,7 @@
 #include <memory>
 #include <thread>
 
+#include "rt_prep.hpp"
 #include "rtg_internal.hpp"
 
 namespace rtg {
@@ -213,9 +214,10 @@ void ktime_end(Arena &A, hipStream_t s, int rec, const unsigned *dev_items) {
     }
 }
 
-// Render one frame into device outputs; the caller holds A.lock.
+// Render one frame into device outputs; the caller holds A.lock.  `cancel`:
+// the job's cancel flag (asynchronous jobs), else null.
 static int render_device(SceneRes &S, const rtg_params &p, int W, int H, const RenderOut &o, hipStream_t stream,
-                         Arena &A) {
+                         Arena &A, const Cancel *cancel) {
     const size_t npix = (size_t)W * H;
     A.ktimes = (p.flags & RTG_FLAG_KERNEL_TIMES) != 0;
     if (A.ktimes && A.kitems_host == nullptr)
@@ -241,8 +243,8 @@ static int render_device(SceneRes &S, const rtg_params &p, int W, int H, const R
         return fail(RTG_E_INVALID, "the FP64 near-tie re-test needs the wavefront engine");
     int st = A.ws.order_before(stream);
     if (st != RTG_OK) return st;
-    st = engine == RTG_ENGINE_WAVEFRONT ? launch_wavefront(S, p, W, H, o, stream, A.ws, A, A.cancel_host)
-                                        : launch_megakernel(S, p, W, H, o, stream, A);
+    st = engine == RTG_ENGINE_WAVEFRONT ? launch_wavefront(S, p, W, H, o, stream, A.ws, A, cancel)
+                                        : launch_megakernel(S, p, W, H, o, stream, A, cancel);
     const int st2 = A.ws.mark_done(stream);
     return st != RTG_OK ? st : st2;
 }
@@ -255,7 +257,6 @@ struct rtg_job_s {
     std::thread th;
     std::atomic<int> status{RTG_E_PENDING};
     std::atomic<int> cancel{0};
-    std::atomic<int> running{0};
     rtg_stats stats{};
 };
 
@@ -489,16 +490,37 @@ static int render_host(rtg_scene_t scene, const rtg_params *params, uint8_t *rgb
     const size_t npix = (size_t)W * H;
     if (ensure(A.out_rgb, npix * 3) || (ids && ensure(A.out_ids, npix * 8)) || (tprim && ensure(A.out_t, npix * 4)))
         return fail(RTG_E_NOMEM, "output buffers");
-    if (job) {
-        job->running = 1;
-        *A.cancel_host = job->cancel.load();
-    } else {
-        *A.cancel_host = 0;
-    }
+    // The arena's mapped cancel flag belongs to this job while it renders:
+    // rtg_cancel sets it only for the arena's current job, under cancel_lock,
+    // and it is cleared when the job ends.
+    struct JobFlag {  // owns the flag for the job's render, on every return path
+        Arena &A;
+        rtg_job_s *job;
+        bool seen = false;  // the flag was set when the job gave it up
+        JobFlag(Arena &a, rtg_job_s *j) : A(a), job(j) {
+            if (!job) return;
+            std::lock_guard<std::mutex> c(A.cancel_lock);
+            A.cancel_job = job;
+            *A.cancel_host = job->cancel.load();
+        }
+        void release() {
+            if (!job) return;
+            std::lock_guard<std::mutex> c(A.cancel_lock);
+            seen = seen || *A.cancel_host != 0;
+            *A.cancel_host = 0;
+            A.cancel_job = nullptr;
+        }
+        ~JobFlag() { release(); }
+    } flag(A, job);
+    // A cancellable frame renders in passes of at most 2^19 samples (whole
+    // pixel-block rows): a cancel stops the pass in flight, and the passes
+    // already done keep their pixels, as the reference's workers keep the
+    // tiles they finished.
+    const Cancel cancel{A.cancel_host, A.cancel_dev, size_t(1) << 19};
     RenderOut o{A.out_rgb.as<uint8_t>(), ids ? A.out_ids.as<int32_t>() : nullptr, tprim ? A.out_t.as<float>() : nullptr,
                 A.counters.as<unsigned long long>(), A.counters.as<unsigned long long>() + 4};
     RTG_HIP(hipEventRecord(A.ev0, A.stream));
-    st = render_device(scene->res, *params, W, H, o, A.stream, A);
+    st = render_device(scene->res, *params, W, H, o, A.stream, A, job ? &cancel : nullptr);
     RTG_HIP(hipEventRecord(A.ev1, A.stream));
     // Results through the arena's pinned staging: one device-to-host copy per
     // array, then host copies into the caller's (pageable) arrays.
@@ -518,13 +540,12 @@ static int render_host(rtg_scene_t scene, const rtg_params *params, uint8_t *rgb
                                 {tprim, A.pinned_out + o_t, b_t}};
         copy_out(cp, 3);
     }
-    if (job) job->running = 0;
+    flag.release();
+    const bool cancelled = flag.seen;
     if (st != RTG_OK) return st;
     if (pst != RTG_OK) return pst;
     float ms = 0;
     RTG_HIP(hipEventElapsedTime(&ms, A.ev0, A.ev1));
-    const bool cancelled = *A.cancel_host != 0;
-    *A.cancel_host = 0;
     if (stats) {
         stats->rays_primary = cnt[0];
         stats->rays_shadow = cnt[1];
@@ -557,7 +578,7 @@ int rtg_render_device(rtg_scene_t scene, const rtg_params *params, uint8_t *rgb_
     unsigned long long *counts =
         stats_dev ? reinterpret_cast<unsigned long long *>(stats_dev) : A.counters.as<unsigned long long>();
     RenderOut o{rgb_dev, ids_dev, tprim_dev, counts, A.counters.as<unsigned long long>() + 4};
-    return render_device(scene->res, *params, W, H, o, s, A);
+    return render_device(scene->res, *params, W, H, o, s, A, nullptr);
 }
 
 int rtg_render_async(rtg_scene_t scene, const rtg_params *params, uint8_t *rgb, int32_t *ids, float *tprim,
@@ -581,7 +602,10 @@ int rtg_poll(rtg_job_t job) {
 int rtg_cancel(rtg_job_t job) {
     if (job == nullptr) return fail(RTG_E_INVALID, "job is NULL");
     job->cancel = 1;
-    if (job->running.load() && arena()) *arena()->cancel_host = 1;
+    if (Arena *A = arena()) {
+        std::lock_guard<std::mutex> c(A->cancel_lock);
+        if (A->cancel_job == job) *A->cancel_host = 1;
+    }
     return RTG_OK;
 }
 
@@ -675,9 +699,30 @@ int rtg_render_batch(const rtg_request *reqs, int n, rtg_stats *stats) {
     if (ids_bytes) k_fill_u8<<<1024, 256, 0, s>>>(A.out_ids.as<uint8_t>(), ids_bytes, 0xff);
     k_zero_u64<<<1, 64, 0, s>>>(counts, 5);
     RTG_HIP(hipEventRecord(A.ev0, s));
+    // One wavefront level holds at most level_ray_cap rays: a request list
+    // with more level-0 samples runs as consecutive batches, and a request
+    // beyond the cap on its own as a frame in passes.
+    int n_on = 0;
+    std::vector<size_t> samples(n);
+    for (int i = 0; i < n; ++i) {
+        n_on = std::max(n_on, __builtin_popcount(items[i].scene->lights_on));
+        samples[i] = frame_samples(items[i].W, items[i].H, items[i].params.spp, A.tune.tile_lg);
+    }
+    const size_t cap = level_ray_cap(n_on);
     int replayed = 0;
     int st = A.ws.order_before(s);
-    if (st == RTG_OK) st = launch_batch(items.data(), n, counts, counts + 4, s, A.ws, A, eager ? 0 : 1, &replayed);
+    for (int b = 0; st == RTG_OK && b < n;) {
+        const int e = batch_group_end(samples.data(), n, b, cap);
+        if (e == b + 1 && batch_bucket(samples[b]) > cap) {
+            const RenderOut o{items[b].rgb, items[b].ids, nullptr, counts, counts + 4};
+            st = launch_wavefront(*items[b].scene, items[b].params, items[b].W, items[b].H, o, s, A.ws, A, nullptr);
+        } else {
+            int rep = 0;
+            st = launch_batch(items.data() + b, e - b, counts, counts + 4, s, A.ws, A, eager ? 0 : 1, &rep);
+            replayed |= rep;
+        }
+        b = e;
+    }
     if (st == RTG_OK) st = A.ws.mark_done(s);
     RTG_HIP(hipEventRecord(A.ev1, s));
     // Results: one copy of the whole output region into pinned memory, then a

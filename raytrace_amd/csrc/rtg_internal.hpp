@@ -147,8 +147,10 @@ struct Arena {
     size_t pinned_out_bytes = 0;
     Workspace ws;
     std::vector<std::unique_ptr<ServeSlot>> serve_slots;
-    int *cancel_host = nullptr;  // pinned, mapped
+    int *cancel_host = nullptr;  // pinned, mapped: the cancel flag of the job rendering now
     int *cancel_dev = nullptr;
+    std::mutex cancel_lock;        // guards cancel_job and writes of *cancel_host
+    const void *cancel_job = nullptr;  // the asynchronous job the flag belongs to, or null
     // kernel timing (RTG_FLAG_KERNEL_TIMES)
     struct KernelRec {
         int kind;
@@ -201,11 +203,25 @@ struct RenderOut {
     unsigned long long *retests = nullptr;  // device, 1 counter: FP64 re-tests (optional)
 };
 
-// Launch one frame of the megakernel on `stream` (asynchronous).
+// Cooperative cancel of a frame (rtg_cancel of an asynchronous job).  `host`
+// and `dev` are the host and device addresses of one mapped flag.  Kernels
+// poll `dev`; the host checks `host` between levels and passes.  A wavefront
+// frame that sees it stops in its current pass and leaves that pass's pixels
+// untouched; passes of at most `pass_samples` level-0 samples (0: no limit
+// beyond the engine's own) make the cancel take effect part-way through a
+// frame, as the reference's workers stop between pixels.
+struct Cancel {
+    const int *host;
+    const int *dev;
+    size_t pass_samples;
+};
+
+// Launch one frame of the megakernel on `stream` (asynchronous); `cancel`
+// may be null.
 int launch_megakernel(const SceneRes &S, const rtg_params &p, int W, int H, const RenderOut &o, hipStream_t stream,
-                      Arena &A);
+                      Arena &A, const Cancel *cancel);
 // Launch one frame of the wavefront engine on `stream` (asynchronous) with
-// the buffers of `ws`; `cancel_host` (optional) is polled between levels.
+// the buffers of `ws`; `cancel` may be null.
 // Host copies of device results staged in pinned memory, run on up to 8
 // host threads in pieces of at most 1 MB.
 struct HostCopy {
@@ -224,7 +240,7 @@ int reserve_frame(Workspace &ws, int W, int H, int spp, int max_level, int fanou
 int reserve_batch(Workspace &ws, size_t samples, int n_reqs, int n_scenes, size_t table_floats, int max_level,
                   int fanout, int n_on, hipStream_t stream, Arena &A);
 int launch_wavefront(const SceneRes &S, const rtg_params &p, int W, int H, const RenderOut &o, hipStream_t stream,
-                     Workspace &ws, Arena &A, const int *cancel_host);
+                     Workspace &ws, Arena &A, const Cancel *cancel);
 
 // One request of a batch (rtg_render_batch): a frame of `scene` into device
 // outputs.

@@ -57,7 +57,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_megakernel(SceneView S, MegaA
     const ChunkRange cr = xcd_chunks(A.nblocks);
     const int tw = (A.W >> 6) << 6, th = (A.H >> 6) << 6;
     for (int pb = cr.first; pb < cr.end; pb += cr.step) {
-        if (__builtin_nontemporal_load(A.cancel) != 0) break;
+        if (A.cancel != nullptr && __builtin_nontemporal_load(A.cancel) != 0) break;
         const int bx = pb % A.bw, by = pb / A.bw;
         const int px = bx * 16 + ((tid >> 6) << 3) + (tid & 7), py = by * 8 + ((tid & 63) >> 3);
         if (px >= A.W || py >= A.H) continue;
@@ -136,7 +136,7 @@ int upload_raygen_tables(const SceneRes &S, int W, int H, hipStream_t stream, Wo
 int persistent_blocks(const Arena &A) { return std::max(8, (A.cus > 0 ? A.cus : 256) * 8); }
 
 int launch_megakernel(const SceneRes &S, const rtg_params &p, int W, int H, const RenderOut &o, hipStream_t stream,
-                      Arena &A) {
+                      Arena &A, const Cancel *cancel) {
     double dp;
     int st = upload_raygen_tables(S, W, H, stream, A.ws, dp);
     if (st != RTG_OK) return st;
@@ -153,7 +153,7 @@ int launch_megakernel(const SceneRes &S, const rtg_params &p, int W, int H, cons
     a.by = A.ws.tables.as<float>() + W;
     a.rgb = o.rgb, a.ids = o.ids, a.tprim = o.tprim;
     a.counts = o.counts;
-    a.cancel = A.cancel_dev;
+    a.cancel = cancel ? cancel->dev : nullptr;
     a.spill = A.ws.spill.as<int>();
     const int kt = ktime_begin(A, stream, KT_MEGA, (uint64_t)W * H * a.spp);
     k_megakernel<<<blocks, kBlockThreads, 0, stream>>>(S.view, a);

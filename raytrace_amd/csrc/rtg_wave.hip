@@ -60,6 +60,9 @@ struct LevelCtr {
     LineCtr sched[3][8];  // WaveQueue counters of extend, shade, occlude
 };
 constexpr int kCtrWords = (int)(sizeof(LevelCtr) / sizeof(unsigned));
+// The counters of a pass: one LevelCtr per level, then the pass's abort word
+// (non-zero once a kernel of the pass stopped for a cancel).
+constexpr int kPassCtrWords = Workspace::kLevels * kCtrWords + (int)(sizeof(LineCtr) / sizeof(unsigned));
 
 // One request's render parameters: a kernel argument (B = false) or an entry
 // of the batch's request table (B = true).
@@ -91,7 +94,21 @@ struct WaveArgs {
     unsigned long long *counts;
     unsigned long long *retests;  // closest-hit queries re-tested in FP64
     int *spill;
+    const int *cancel;  // mapped cancel flag of an asynchronous job, or null
+    unsigned *abort;    // the pass's abort word
 };
+
+// Cooperative cancel: a wave that finds the job's cancel flag set marks the
+// pass aborted and takes no work.  Called by every lane of a full wave before
+// its work loop (wave-uniform result; no barrier follows it).
+__device__ inline bool wave_cancelled(const WaveArgs &A) {
+    if (A.cancel == nullptr) return false;
+    int c = 0;
+    if (__lane_id() == 0) c = __builtin_nontemporal_load(A.cancel);
+    c = __builtin_amdgcn_readfirstlane(c);
+    if (c != 0 && __lane_id() == 0) atomicOr(A.abort, 1u);
+    return c != 0;
+}
 
 template <bool B>
 __device__ inline const WaveReq &req_at(const WaveArgs &A, int r) {
@@ -218,6 +235,7 @@ __global__ __launch_bounds__(kBlockThreads, 6) void k_extend(SceneView S0, WaveA
         __shared__ BvhNode top_nodes[kTopNodes];
         stack_top_nodes(st, top_nodes, S0);
     }
+    if (wave_cancelled(A)) return;
     WaveQueue wq(sched, n, A.grab_lg);
     uint32_t c_prim = 0, c_refl = 0, c_refr = 0;
     int n_retest = 0;
@@ -296,6 +314,7 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_shade(SceneView S0, WaveAr
                                                          unsigned *masks, WRay *next, LevelCtr *ctr,
                                                          unsigned *shadow, float4 *slots) {
     const int n = k == 0 ? A.n_samples : (int)*n_ptr;
+    if (wave_cancelled(A)) return;
     WaveQueue wq(ctr->sched[1], n, A.grab_lg);
     uint32_t c_shadow = 0;
     for (int base = wq.next(); base >= 0; base = wq.next()) {
@@ -431,6 +450,7 @@ __global__ __launch_bounds__(kBlockThreads, 8) void k_occlude(SceneView S0, Wave
         __shared__ BvhNode top_nodes[kTopNodes];
         stack_top_nodes(st, top_nodes, S0);
     }
+    if (wave_cancelled(A)) return;
     WaveQueue wq(ctr->sched[2], n, A.grab_lg);
     for (int base = wq.next(); base >= 0; base = wq.next()) {
         const int j = base + __lane_id();
@@ -458,6 +478,7 @@ __global__ __launch_bounds__(kBlockThreads, 8) void k_combine(SceneView S0, Wave
                                                            const unsigned *masks, const float4 *slots,
                                                            float4 *parent_slots, float4 *samples) {
     const int n = k == 0 ? A.n_samples : (int)*n_ptr;
+    if (wave_cancelled(A)) return;
     const ChunkRange cr = xcd_chunks((n + kBlockThreads - 1) / kBlockThreads);
     for (int c = cr.first; c < cr.end; c += cr.step) {
         const int i = c * kBlockThreads + threadIdx.x;
@@ -489,6 +510,8 @@ __global__ __launch_bounds__(kBlockThreads, 8) void k_combine(SceneView S0, Wave
 // 3DElement.cpp:463-468).  Grid y = request.
 template <bool B>
 __global__ void k_resolve(WaveArgs A, const float4 *samples) {
+    // a pass that a cancel stopped part-way keeps its pixels untouched (127)
+    if (A.cancel != nullptr && *(volatile unsigned *)A.abort != 0) return;
     const WaveReq &R = req_at<B>(A, blockIdx.y);
     // pixels of the pass: block rows row0 .. row0 + n_samples / (bw 64 spp)
     const int lg = R.tile_lg, th_lg = 6 - lg;
@@ -585,7 +608,9 @@ static size_t per_ray_bytes(int n_on) {
 
 size_t batch_queue_bytes(size_t samples, int max_level, int fanout, int n_on, size_t budget) {
     size_t need = 0, b = samples;
+    const size_t cap = level_ray_cap(n_on);
     for (int k = 0; k <= max_level && b > 0; ++k) {
+        if (b > cap) return 0;
         need += b * per_ray_bytes(n_on);
         if (need > budget) return 0;
         b *= (size_t)fanout;
@@ -596,13 +621,15 @@ size_t batch_queue_bytes(size_t samples, int max_level, int fanout, int n_on, si
 static WavePlan make_plan(int n_samples, int max_level, int fanout, int n_on, int n_reqs, size_t budget) {
     WavePlan pl{};
     pl.n_samples = n_samples, pl.max_level = max_level, pl.fanout = fanout, pl.n_on = n_on, pl.n_reqs = n_reqs;
-    const size_t per_ray = per_ray_bytes(n_on);
+    const size_t per_ray = per_ray_bytes(n_on), cap = level_ray_cap(n_on);
     size_t need = 0;
     pl.no_sync = true;
     for (int k = 0; k <= max_level; ++k) {
         pl.bound[k] = k == 0 ? (size_t)n_samples : std::min<size_t>(pl.bound[k - 1] * (size_t)fanout, size_t(1) << 40);
         need += pl.bound[k] * per_ray;
-        if (need > budget) pl.no_sync = false;
+        // a bound past the level cap: the levels are sized from their real
+        // counts instead, each checked against the cap before it runs
+        if (need > budget || pl.bound[k] > cap) pl.no_sync = false;
     }
     return pl;
 }
@@ -629,7 +656,7 @@ static int reserve_all(Workspace &ws, const WavePlan &pl, int cus) {
     if (ws.grow(ws.spill, spill_ints(gmax) * sizeof(int))) return fail(RTG_E_NOMEM, "spill");
     if (ws.grow(ws.spill2, spill_ints(std::max(g0.occlude, g1.occlude)) * sizeof(int)))
         return fail(RTG_E_NOMEM, "spill");
-    if (ws.grow(ws.w_qcount, (size_t)Workspace::kLevels * sizeof(LevelCtr))) return fail(RTG_E_NOMEM, "counters");
+    if (ws.grow(ws.w_qcount, (size_t)kPassCtrWords * sizeof(unsigned))) return fail(RTG_E_NOMEM, "counters");
     if (ws.grow(ws.w_samples, (size_t)pl.n_samples * sizeof(float4))) return fail(RTG_E_NOMEM, "samples");
     for (int k = 0; k <= pl.max_level && pl.bound[k] > 0; ++k) {
         const int st = reserve_level(ws, pl, k, pl.bound[k]);
@@ -643,13 +670,14 @@ static int reserve_all(Workspace &ws, const WavePlan &pl, int cus) {
 // allocates or waits.
 template <bool B>
 static int enqueue_wave(const SceneView &S0, const WaveArgs &a, const WavePlan &pl, hipStream_t stream,
-                        Workspace &ws, Arena &A, const int *cancel_host, bool retest, bool overlap) {
+                        Workspace &ws, Arena &A, const Cancel *cancel, bool retest, bool overlap) {
+    const int *cancel_host = cancel ? cancel->host : nullptr;
     const int cus = A.cus > 0 ? A.cus : 256;
     const Grids<B> g(cus, A.tune.grid_pct);
     if (!pl.no_sync) {
         if (ws.grow(ws.spill, spill_ints(std::max(g.extend, g.occlude)) * sizeof(int)))
             return fail(RTG_E_NOMEM, "spill");
-        if (ws.grow(ws.w_qcount, (size_t)Workspace::kLevels * sizeof(LevelCtr)))
+        if (ws.grow(ws.w_qcount, (size_t)kPassCtrWords * sizeof(unsigned)))
             return fail(RTG_E_NOMEM, "queue counters");
     }
     // The shadow rays of level k (k_occlude) depend only on
@@ -672,19 +700,26 @@ static int enqueue_wave(const SceneView &S0, const WaveArgs &a, const WavePlan &
         return RTG_OK;
     };
     LevelCtr *ctr = ws.w_qcount.as<LevelCtr>();
-    const int ctr_words = Workspace::kLevels * kCtrWords;
-    k_zero_u32<<<(ctr_words + 255) / 256, 256, 0, stream>>>(ws.w_qcount.as<unsigned>(), ctr_words);
+    k_zero_u32<<<(kPassCtrWords + 255) / 256, 256, 0, stream>>>(ws.w_qcount.as<unsigned>(), kPassCtrWords);
     auto grid = [](int gmax, size_t items) {
         return (int)std::max<size_t>(8, std::min<size_t>(gmax, (items + 127) / 128));
     };
     WaveArgs args = a;
     args.spill = ws.spill.as<int>();
+    args.cancel = cancel ? cancel->dev : nullptr;
+    args.abort = ws.w_qcount.as<unsigned>() + Workspace::kLevels * kCtrWords;
     WaveArgs oargs = args;
     if (ov) oargs.spill = ws.spill2.as<int>();
     size_t n_k = (size_t)pl.n_samples;
     int last = -1;
+    const size_t cap = level_ray_cap(pl.n_on);
     for (int k = 0; k <= pl.max_level && n_k > 0; ++k) {
         if (cancel_host && *cancel_host) return join();
+        if (n_k > cap) {
+            (void)join();
+            return fail(RTG_E_LIMIT, "wavefront level " + std::to_string(k) + " holds " + std::to_string(n_k) +
+                                         " rays, more than the per-level limit of " + std::to_string(cap));
+        }
         if (!pl.no_sync) {
             const int st = reserve_level(ws, pl, k, n_k);
             if (st != RTG_OK) {
@@ -782,7 +817,8 @@ int reserve_frame(Workspace &ws, int W, int H, int spp, int max_level, int fanou
 }
 
 int launch_wavefront(const SceneRes &S, const rtg_params &p, int W, int H, const RenderOut &o, hipStream_t stream,
-                     Workspace &ws, Arena &A, const int *cancel_host) {
+                     Workspace &ws, Arena &A, const Cancel *cancel) {
+    const int *cancel_host = cancel ? cancel->host : nullptr;
     double dp;
     int st = upload_raygen_tables(S, W, H, stream, ws, dp);
     if (st != RTG_OK) return st;
@@ -809,19 +845,21 @@ int launch_wavefront(const SceneRes &S, const rtg_params &p, int W, int H, const
     // run in passes of whole block rows, at most kPassSamples level-0
     // samples each, so that the queues of all levels stay a few GB.
     const int rows = (H + th - 1) / th, per_row = r.bw * 64 * r.spp;
-    const int rows_per_pass = S.max_fanout >= 2 ? std::max(1, kPassSamples / per_row) : rows;
+    const int n_on = __builtin_popcount(S.lights_on);
+    int rows_per_pass = frame_rows_per_pass(rows, (size_t)per_row, S.max_fanout, n_on, kPassSamples);
+    if (cancel && cancel->pass_samples > 0)
+        rows_per_pass = std::min(rows_per_pass, std::max(1, (int)(cancel->pass_samples / (size_t)per_row)));
     for (int row0 = 0; row0 < rows; row0 += rows_per_pass) {
         r.row0 = row0;
         r.n_samples = std::min(rows_per_pass, rows - row0) * per_row;
         a.n_samples = r.n_samples;
-        WavePlan pl = make_plan(r.n_samples, p.max_level, S.max_fanout, __builtin_popcount(S.lights_on), 1,
-                                A.no_sync_budget);
+        WavePlan pl = make_plan(r.n_samples, p.max_level, S.max_fanout, n_on, 1, A.no_sync_budget);
         // A very large frame keeps the GPU busy at every level; there the
         // host round trip per level costs little, and sizing each level from
         // the previous level's count saves the worst-case queue memory.
         if (r.n_samples > kNoSyncMaxSamples) pl.no_sync = false;
         if (pl.no_sync && (st = reserve_all(ws, pl, A.cus > 0 ? A.cus : 256)) != RTG_OK) return st;
-        st = enqueue_wave<false>(S.view, a, pl, stream, ws, A, cancel_host, r.retest != 0,
+        st = enqueue_wave<false>(S.view, a, pl, stream, ws, A, cancel, r.retest != 0,
                                  (p.flags & RTG_FLAG_ONE_STREAM) == 0);
         if (st != RTG_OK || (cancel_host && *cancel_host)) return st;
     }
@@ -855,8 +893,8 @@ int reserve_batch(Workspace &ws, size_t samples, int n_reqs, int n_scenes, size_
     if (st == RTG_OK) st = ws.side_init();
     if (st != RTG_OK) return st;
     if (samples == 0 || n_reqs <= 0) return RTG_OK;
-    if (samples > (size_t)INT32_MAX / 2 || n_reqs > kMaxBatch) return fail(RTG_E_LIMIT, "batch bound too large");
     const size_t padded = batch_bucket(samples);
+    if (padded > level_ray_cap(n_on) || n_reqs > kMaxBatch) return fail(RTG_E_LIMIT, "batch bound too large");
     int n_pad = 1;
     while (n_pad < n_reqs) n_pad <<= 1;
     const WavePlan pl = make_plan((int)padded, max_level, fanout, n_on, n_pad, A.no_sync_budget);
@@ -914,8 +952,11 @@ int launch_batch(const BatchItem *items, int n, unsigned long long *counts, unsi
         r.rgb = it.rgb, r.ids = it.ids, r.tprim = it.tprim;
         total += (size_t)r.n_samples;
     }
-    if (total > (size_t)INT32_MAX / 2) return fail(RTG_E_LIMIT, "batch has too many samples");
     const size_t padded = batch_bucket(total);
+    if (padded > level_ray_cap(n_on))
+        return fail(RTG_E_LIMIT, "batch has more level-0 samples than one wavefront level holds (" +
+                                     std::to_string(padded) + " > " + std::to_string(level_ray_cap(n_on)) +
+                                     "): split it");
     // Level 0 is read in whole 64-sample chunks (chunk_req holds one entry per
     // chunk): a length that is not a multiple of 64 would read past it.
     if (padded % 64 != 0 || padded < total) return fail(RTG_E_STATE, "batch bucket is not whole chunks");

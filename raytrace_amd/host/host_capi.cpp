@@ -153,4 +153,18 @@ int rtg_host_raytrace(rtg_host_scene_t h, int type, int max_level, int frames, u
     return rt.lastStatus;
 }
 
+int rtg_host_raytrace_stop(rtg_host_scene_t h, int type, int max_level, double stop_after_s, uint8_t *out,
+                           int *finished_before_stop) {
+    if (!valid(h) || out == nullptr || !(stop_after_s >= 0.0)) return RTG_E_INVALID;
+    RayTracer rt(*h->scene);
+    rt.maxLevel = (uint8_t)max_level;
+    rt.start((uint8_t)type, 4);
+    std::this_thread::sleep_for(std::chrono::duration<double>(stop_after_s));
+    if (finished_before_stop != nullptr) *finished_before_stop = rt.isFinish ? 1 : 0;
+    rt.stop();
+    while (!rt.isFinish) std::this_thread::sleep_for(std::chrono::milliseconds(1));
+    memcpy(out, rt.output, 2048 * 2048 * 3);
+    return rt.lastStatus;
+}
+
 }  // extern "C"

@@ -302,4 +302,16 @@ void emul_traffic(uint64_t out[2]) {
 // Level-0 length the batch engine pads `total` samples to (rt_prep.hpp).
 unsigned long long emul_batch_bucket(unsigned long long total) { return (unsigned long long)rtg::batch_bucket(total); }
 
+// Queue limits of one wavefront level and the pass / group planning that
+// keeps every level within them (rt_prep.hpp).
+unsigned long long emul_level_ray_cap(int n_on) { return (unsigned long long)rtg::level_ray_cap(n_on); }
+int emul_frame_rows_per_pass(int rows, unsigned long long per_row, int fanout, int n_on,
+                             unsigned long long branch_pass) {
+    return rtg::frame_rows_per_pass(rows, (size_t)per_row, fanout, n_on, (size_t)branch_pass);
+}
+int emul_batch_group_end(const unsigned long long *samples, int n, int begin, unsigned long long cap) {
+    std::vector<size_t> s(samples, samples + n);
+    return rtg::batch_group_end(s.data(), n, begin, (size_t)cap);
+}
+
 }  // extern "C"

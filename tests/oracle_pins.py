@@ -267,6 +267,174 @@ def pin_quantize(lib):
     assert q(float("nan")) == 0
 
 
+def _frame(rng):
+    """A random orthonormal frame (normal, axisx, axisy) in FP64."""
+    q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+    return q[:, 0], q[:, 1], q[:, 2]
+
+
+def pin_plane_uv(lib):
+    """Plane texture coordinates (Basic3DObject.cpp:390-396): for a hit at
+    P = P0 + a axisx + b axisy the formula f = 1/(axisx.(d x axisy))/5,
+    u = ((o-P0).(d x axisy)) f, v = (d.((o-P0) x axisx)) f reduces to
+    u = a/5, v = b/5 (scalar triple products cancel), whatever the ray."""
+    rng = np.random.default_rng(11)
+    uv = (C.c_float * 2)()
+    n_checked = 0
+    for _ in range(2000):
+        n, ax, ay = _frame(rng)
+        p0 = rng.uniform(-5, 5, 3)
+        a, b = rng.uniform(-20, 20, 2)
+        P = p0 + a * ax + b * ay
+        o = P + n * rng.uniform(0.5, 15) + ax * rng.uniform(-10, 10) + ay * rng.uniform(-10, 10)
+        d = P - o
+        dist = np.linalg.norm(d)
+        d /= dist
+        if abs(d @ n) < 0.05:  # keep off grazing incidence
+            continue
+        t = lib.oracle_plane_hit(f4(*np.float32(o)), f4(*np.float32(d)), f4(*np.float32(p0)), f4(*np.float32(n)),
+                                 f4(*np.float32(ax)), f4(*np.float32(ay)), uv)
+        scale = 1e-4 * (1 + dist) / abs(d @ n)
+        assert abs(t - dist) <= scale, (t, dist)
+        assert abs(uv[0] - a / 5) <= scale and abs(uv[1] - b / 5) <= scale, (list(uv), a / 5, b / 5)
+        n_checked += 1
+    assert n_checked > 1500
+    # the default plane of Scene::AddPlane (y = 0, frame from Plane::rotate)
+    import raytrace_amd as rt
+    from raytrace_amd import _capi
+    hs = rt.HostScene.new()
+    k = hs.add_plane()
+    ob = _capi.desc_of(hs.desc_ptr).objects[k]
+    v3 = lambda q: np.array([q.x, q.y, q.z], np.float64)
+    n, ax, ay, p0 = v3(ob.normal), v3(ob.axisx), v3(ob.axisy), v3(ob.position)
+    assert abs(n @ ax) < 1e-6 and abs(n @ ay) < 1e-6 and abs(ax @ ay) < 1e-6 and abs(n[1] - 1) < 1e-6
+    for x, z in [(1.0, -2.0), (3.5, 4.0), (-7.25, 0.5)]:
+        P = np.array([x, 0.0, z])
+        o = np.array([0.0, 4.0, 15.0])
+        d = (P - o) / np.linalg.norm(P - o)
+        lib.oracle_plane_hit(f4(*o), f4(*np.float32(d)), f4(*p0), f4(*n), f4(*ax), f4(*ay), uv)
+        assert abs(uv[0] - ((P - p0) @ ax) / 5) < 1e-4 and abs(uv[1] - ((P - p0) @ ay) / 5) < 1e-4, (list(uv), P)
+
+
+def pin_texel(lib):
+    """Texel lookup (3DElement.cpp:430-450): texel (x, y) of a w x h texture
+    with x = int(frac(u) w), y = int(frac(v) h), frac wrapping negative
+    coordinates into [0, 1); rows of w texels; bytes in B, G, R order; no
+    texture gives white."""
+    from raytrace_amd import _capi
+    out = (C.c_float * 3)()
+    lib.oracle_texel(None, 0.3, 0.7, out)
+    assert list(out) == [1.0, 1.0, 1.0]
+    for w, h in [(4, 4), (5, 3), (2, 7), (16, 9)]:
+        data = np.zeros((h, w, 3), np.uint8)
+        for y in range(h):
+            for x in range(w):
+                data[y, x] = (10 * x + 1, 10 * y + 2, 200 + x % 3)  # B, G, R
+        buf = np.ascontiguousarray(data.reshape(-1))
+        tex = _capi.rtg_texture(w, h, buf.ctypes.data_as(C.POINTER(C.c_uint8)))
+        for x in range(w):
+            for y in range(h):
+                for ku, kv in [(0, 0), (-1, 2), (3, -3), (-2, -1)]:
+                    u, v = (x + 0.5) / w + ku, (y + 0.5) / h + kv
+                    lib.oracle_texel(C.byref(tex), u, v, out)
+                    want = [np.float32(data[y, x, c]) / np.float32(255) for c in (2, 1, 0)]
+                    assert list(out) == want, (w, h, x, y, u, v, list(out), want)
+
+
+def pin_box(lib):
+    """Box::intersect (Basic3DObject.cpp:274-300): the slab entry distance, and
+    the normal of the face it enters (the coordinate within 1e-6 of the
+    half-extent picks the axis, its sign the side); rays starting inside miss
+    (entry 0 fails t > 1e-6).  Box(l, w, h) (234-240) leaves min at the origin
+    and max negative, so no coordinate matches: the normal is 0/0."""
+    rng = np.random.default_rng(12)
+    nrm = (C.c_float * 4)()
+    # dyadic geometry along the axes: every float step is exact, so the hit
+    # point lies on the face and the pick is decided by the rule alone
+    for _ in range(200):
+        pos = rng.integers(-32, 33, 3) / 8.0
+        half = rng.choice([0.5, 1.0, 1.5, 2.0], 3)
+        for axis in range(3):
+            for sign in (1.0, -1.0):
+                N = np.zeros(3)
+                N[axis] = sign
+                t1, t2 = [k for k in range(3) if k != axis]
+                face = pos + N * half
+                face[t1] += rng.integers(-7, 8) / 16.0 * half[t1]
+                face[t2] += rng.integers(-7, 8) / 16.0 * half[t2]
+                D = rng.integers(4, 80) / 4.0
+                o = face + N * D
+                t = lib.oracle_box_hit(f4(*np.float32(o)), f4(*np.float32(-N)), f4(*np.float32(pos)),
+                                       f4(*np.float32(-half)), f4(*np.float32(half)), nrm)
+                assert abs(t - D) <= 1e-5 * (1 + np.abs(o).max()), (t, D)
+                assert [nrm[0], nrm[1], nrm[2]] == list(N), (axis, sign, list(nrm))
+        inside = pos + rng.uniform(-0.4, 0.4, 3) * half
+        dv = rng.standard_normal(3)
+        t = lib.oracle_box_hit(f4(*np.float32(inside)), f4(*np.float32(dv / np.linalg.norm(dv))),
+                               f4(*np.float32(pos)), f4(*np.float32(-half)), f4(*np.float32(half)), nrm)
+        assert t == np.float32(1e20), t
+    # the 1e-6 band: a hit on the +z face within 1e-6 of the +y face also
+    # picks y (an edge normal); 2e-6 away it does not
+    for dy, want in ((2.0 ** -21, [0.0, 1 / math.sqrt(2), 1 / math.sqrt(2)]), (2.0 ** -19, [0.0, 0.0, 1.0])):
+        t = lib.oracle_box_hit(f4(0.25, 1.0 - dy, 9.0), f4(0, 0, -1), f4(0, 0, 0), f4(-1, -1, -1), f4(1, 1, 1), nrm)
+        assert t == 8.0 and np.allclose([nrm[0], nrm[1], nrm[2]], want, atol=1e-7), (dy, t, list(nrm))
+    # Box(l, w, h): min (0,0,0), max -(l, w, h)/2: the box [-l/2, 0] x .. is hit, the normal is NaN
+    # (an oblique ray; an axis-parallel one is rejected by the parallel-slab
+    # rule, which finds its origin outside [min, max] whatever it is)
+    l, w, h = 2.0, 4.0, 6.0
+    d = np.array([0.3, 0.2, -1.0])
+    d /= np.linalg.norm(d)
+    P = np.array([-0.5, -1.0, 0.0])
+    t = lib.oracle_box_hit(f4(*np.float32(P - 10 * d)), f4(*np.float32(d)), f4(0, 0, 0), f4(0, 0, 0),
+                           f4(-l / 2, -w / 2, -h / 2), nrm)
+    assert abs(t - 10.0) < 1e-5, t
+    assert all(math.isnan(nrm[k]) for k in range(3)), list(nrm)
+    t = lib.oracle_box_hit(f4(-0.5, -1.0, 10.0), f4(0, 0, -1), f4(0, 0, 0), f4(0, 0, 0), f4(-l / 2, -w / 2, -h / 2),
+                           nrm)
+    assert t == np.float32(1e20), t
+
+
+def pin_ballplane(lib):
+    """BallPlane::intersect (Basic3DObject.cpp:486-552): 16 spheres of radius
+    r centred at pos + cx axisx + cy axisy, cx and cy in {-6r, -2r, 2r, 6r}, the
+    cx loop outside, identities this + 1 .. 16 in loop order (index
+    4 i + j); a ray leaving sphere k from inside exits at the far side with the
+    inward normal, rfr 1 for refraction rays (mtl.rfr otherwise), the inside
+    flag flipped; a shadow ray inside sphere k is blocked at distance r."""
+    rng = np.random.default_rng(13)
+    oi = (C.c_int32 * 2)()
+    of = (C.c_float * 4)()
+    for _ in range(40):
+        n, ax, ay = _frame(rng)
+        pos = rng.uniform(-3, 3, 3)
+        r = float(np.float32(rng.uniform(0.2, 1.0)))
+        frame = [f4(*np.float32(pos)), f4(*np.float32(ax)), f4(*np.float32(ay))]
+        for i in range(4):
+            for j in range(4):
+                c = pos + ax * r * (-6 + 4 * i) + ay * r * (-6 + 4 * j)
+                D = rng.uniform(r + 1, 10)
+                o = c + n * D
+                t = lib.oracle_ballplane_hit(f4(*np.float32(o)), f4(*np.float32(-n)), *frame, r, 1.5, -1, 0, 1, oi,
+                                             of)
+                assert abs(t - (D - r)) <= 2e-5 * (1 + D), (i, j, t, D - r)
+                assert oi[0] == 4 * i + j and oi[1] == 1, (i, j, list(oi))
+                assert np.allclose(list(of)[:3], n, atol=1e-4) and of[3] == np.float32(1.5)
+                # from inside sphere 4 i + j, at its centre, along a random direction
+                dv = rng.standard_normal(3)
+                dv /= np.linalg.norm(dv)
+                for ray_type, rfr in ((4, 1.0), (3, 1.5)):  # refraction, reflection
+                    t = lib.oracle_ballplane_hit(f4(*np.float32(c)), f4(*np.float32(dv)), *frame, r, 1.5, 4 * i + j,
+                                                 1, ray_type, oi, of)
+                    if t < 0.999 * r:  # a neighbour is not in the way (spheres are 4r apart): the exit
+                        raise AssertionError((i, j, t, r))
+                    assert abs(t - r) <= 2e-5 * (1 + r), (t, r)
+                    assert oi[0] == 4 * i + j and oi[1] == 0
+                    assert np.allclose(list(of)[:3], -dv, atol=1e-4) and of[3] == np.float32(rfr)
+                t = lib.oracle_ballplane_hit(f4(*np.float32(c)), f4(*np.float32(dv)), *frame, r, 1.5, 4 * i + j, 1,
+                                             2, oi, of)
+                assert t == np.float32(r), (t, r)
+
+
 # --------------------------------------------------- scenes with known answers
 
 
@@ -464,6 +632,6 @@ def pin_debug_modes(lib):
     assert np.allclose(got, [v, v, v]) and out["stats"].rays() == 0, got
 
 
-ALL_PINS = [pin_dot_is_dpps, pin_normalize_is_sse, pin_raygen, pin_sphere, pin_plane, pin_triangle,
-            pin_border_tests, pin_quantize, pin_local_shading, pin_shadow, pin_reflection, pin_refraction_beer,
-            pin_tie_rule, pin_ray_counts, pin_debug_modes]
+ALL_PINS = [pin_dot_is_dpps, pin_normalize_is_sse, pin_raygen, pin_sphere, pin_plane, pin_plane_uv, pin_texel,
+            pin_box, pin_ballplane, pin_triangle, pin_border_tests, pin_quantize, pin_local_shading, pin_shadow,
+            pin_reflection, pin_refraction_beer, pin_tie_rule, pin_ray_counts, pin_debug_modes]

@@ -110,6 +110,70 @@ def test_batch_bucket_is_whole_chunks():
     assert f(2325760) == f(2325760 + 64)
 
 
+def test_wavefront_levels_stay_within_the_entry_packing():
+    """Regression: a shadow entry packs (node << 5 | light) into 32 bits and is
+    appended at a 32-bit signed offset, so no level may hold 2^27 rays or 2^31
+    shadow entries.  A 16384 x 16384 frame at 1 spp (268 M samples, no
+    branching) used to run as one pass, and a batch of 256 requests of
+    1024 x 768 (201 M samples) as one wavefront: node indices wrapped.  The
+    pass and group planning must keep every level-0 queue within the cap
+    (deeper levels of a branching frame are checked as they are counted)."""
+    import ctypes as C
+    from conftest import emul_lib
+    L = emul_lib()
+    cap = L.emul_level_ray_cap
+    cap.argtypes, cap.restype = [C.c_int], C.c_ulonglong
+    rpp = L.emul_frame_rows_per_pass
+    rpp.argtypes, rpp.restype = [C.c_int, C.c_ulonglong, C.c_int, C.c_int, C.c_ulonglong], C.c_int
+    ge = L.emul_batch_group_end
+    ge.argtypes, ge.restype = [C.POINTER(C.c_ulonglong), C.c_int, C.c_int, C.c_ulonglong], C.c_int
+    bucket = L.emul_batch_bucket
+    bucket.argtypes, bucket.restype = [C.c_ulonglong], C.c_ulonglong
+    for n_on in (0, 1, 4, 8, 16, 32):
+        c = cap(n_on)
+        assert c < 2 ** 27 and c * max(1, n_on) < 2 ** 31, (n_on, c)
+    # frames up to the size limit, every tile shape, spp 1 and 4, any fanout
+    for W, H in ((16384, 16384), (16384, 64), (1920, 1080), (64, 16384)):
+        for tile_lg in (2, 3, 4, 5):
+            tw, th = 1 << tile_lg, 64 >> tile_lg
+            rows, per_row_1 = (H + th - 1) // th, ((W + tw - 1) // tw) * 64
+            for spp in (1, 4):
+                per_row = per_row_1 * spp
+                for fanout in (0, 1, 2):
+                    for n_on in (1, 8, 32):
+                        r = rpp(rows, per_row, fanout, n_on, 1 << 21)
+                        assert 1 <= r <= rows
+                        samples = r * per_row
+                        assert samples <= cap(n_on), (W, H, tile_lg, spp, fanout, n_on, r)
+                        assert (samples << 5) < 2 ** 32 and samples * n_on < 2 ** 31
+                        if fanout >= 2:
+                            assert samples <= 1 << 21
+                        elif r < rows:  # a pass is never smaller than the cap requires
+                            assert (r + 1) * per_row > cap(n_on)
+    # the 16384 x 16384 spp 1 frame of the regression: several passes
+    assert rpp(16384 // 8, (16384 // 8) * 64, 1, 32, 1 << 21) < 16384 // 8
+    # batches: consecutive groups within the cap, covering every request once
+    for samples in ([1024 * 768] * 256, [64 * 64] * 4096, [16384 * 16384 * 4] + [1024 * 768] * 3,
+                    list(np.random.default_rng(7).integers(4096, 1024 * 768, 600))):
+        n = len(samples)
+        arr = (C.c_ulonglong * n)(*[int(s) for s in samples])
+        for n_on in (1, 8):
+            b, groups = 0, []
+            while b < n:
+                e = ge(arr, n, b, cap(n_on))
+                assert b < e <= n
+                if e - b > 1 or bucket(samples[b]) <= cap(n_on):
+                    assert bucket(sum(samples[b:e])) <= cap(n_on), (b, e)
+                    if e < n:  # maximal: the next request would not fit
+                        assert bucket(sum(samples[b:e + 1])) > cap(n_on)
+                groups.append((b, e))
+                b = e
+            assert groups[0][0] == 0 and groups[-1][1] == n
+    # 256 x 1024 x 768 needs two batches at 8 lights
+    arr = (C.c_ulonglong * 256)(*([1024 * 768] * 256))
+    assert ge(arr, 256, 0, cap(8)) < 256
+
+
 @pytest.mark.parametrize("workload", [1, 2])
 def test_fp64_retest_changes_only_band_pixels(workload):
     """RTG_FLAG_FP64_RETEST (device code on the CPU): near-tie closest hits are

@@ -104,7 +104,7 @@ def test_async_and_cancel(rt_gpu):
     rt = rt_gpu
     from raytrace_amd import _capi
     lib = _capi.load()
-    log("case: rtg_render_async + rtg_poll + rtg_job_wait, then a cancelled job")
+    log("case: rtg_render_async + rtg_poll + rtg_job_wait, then a job cancelled before it starts")
     hs = rt.HostScene.workload(1)
     ds = rt.DeviceScene(hs)
     p = rt.params(640, 480, max_level=3)
@@ -117,16 +117,101 @@ def test_async_and_cancel(rt_gpu):
     lib.rtg_job_free(job)
     ref = ds.render(p)["rgb"]
     assert (rgb == ref).all()
-    # a job cancelled before it starts renders nothing: every pixel stays 127
+    # the next frame of the arena is not affected by a job cancelled while it
+    # waited for another job: it renders nothing, every pixel stays 127
+    import time
+    hl = rt.HostScene.workload(5, width=1024, height=576, detail=8, n_spheres=64)
+    dl = rt.DeviceScene(hl)
+    pl = rt.params(1024, 576, max_level=8, spp=4)
+    t0 = time.perf_counter()
+    full = dl.render(pl, ids=False, tprim=False)["rgb"]
+    frame_s = time.perf_counter() - t0
+    log(f"  long frame {frame_s * 1e3:.1f} ms")
+    assert frame_s > 0.03
+    rgb_a = np.zeros_like(full)
+    job_a, job_b = C.c_void_p(), C.c_void_p()
+    _capi.check(lib.rtg_render_async(dl.h, C.byref(pl), rgb_a.ctypes.data_as(C.c_void_p), None, None,
+                                     C.byref(job_a)), "rtg_render_async")
+    time.sleep(frame_s / 4)  # job a holds the arena
     rgb2 = np.zeros_like(rgb)
-    _capi.check(lib.rtg_render_async(ds.h, C.byref(p), rgb2.ctypes.data_as(C.c_void_p), None, None, C.byref(job)),
+    _capi.check(lib.rtg_render_async(ds.h, C.byref(p), rgb2.ctypes.data_as(C.c_void_p), None, None,
+                                     C.byref(job_b)), "rtg_render_async")
+    lib.rtg_cancel(job_b)
+    rc_a, rc_b = lib.rtg_job_wait(job_a, None), lib.rtg_job_wait(job_b, None)
+    lib.rtg_job_free(job_a)
+    lib.rtg_job_free(job_b)
+    assert rc_a == 0 and (rgb_a == full).all()
+    assert rc_b == -4 and (rgb2 == 127).all()
+    assert (ds.render(p)["rgb"] == ref).all()
+
+
+def _cancel_mid_frame(rt, ds, p, full, frame_s):
+    """Start an asynchronous render, cancel it a third of the way through while
+    rtg_poll still says pending, and return (status, rgb)."""
+    import time
+    from raytrace_amd import _capi
+    lib = _capi.load()
+    rgb = np.zeros_like(full)
+    job = C.c_void_p()
+    _capi.check(lib.rtg_render_async(ds.h, C.byref(p), rgb.ctypes.data_as(C.c_void_p), None, None, C.byref(job)),
                 "rtg_render_async")
+    time.sleep(frame_s / 3)
+    pending = lib.rtg_poll(job) == 1
     lib.rtg_cancel(job)
     rc = lib.rtg_job_wait(job, None)
     lib.rtg_job_free(job)
-    assert rc in (0, -4)
-    if rc == -4:
-        assert (rgb2 == 127).mean() > 0.0
+    return pending, rc, rgb
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_cancel_mid_frame(rt_gpu, engine):
+    """rtg_cancel / RayTracer::stop (RayTracer.cpp:29-33, 698-701) on a frame in
+    flight: the job reports RTG_E_CANCELLED, pixels rendered before the cancel
+    equal the full render, and the rest keep the value 127."""
+    import time
+    rt = rt_gpu
+    log(f"case: cancel mid-frame, workload 5 at 1024x576, 4 spp, engine {engine}")
+    hs = rt.HostScene.workload(5, width=1024, height=576, detail=8, n_spheres=64)
+    ds = rt.DeviceScene(hs)
+    ml = 8 if engine == 2 else 3
+    p = rt.params(1024, 576, max_level=ml, spp=4, engine=engine)
+    t0 = time.perf_counter()
+    full = ds.render(p, ids=False, tprim=False)["rgb"]
+    frame_s = time.perf_counter() - t0
+    log(f"  full frame {frame_s * 1e3:.1f} ms")
+    assert frame_s > 0.03, "frame too short to cancel part-way"
+    pending, rc, rgb = _cancel_mid_frame(rt, ds, p, full, frame_s)
+    assert pending, "the job finished before the cancel"
+    assert rc == -4, rc
+    same = (rgb == full).all(-1)
+    blank = (rgb == 127).all(-1)
+    log(f"  rendered before the cancel: {same.mean():.3f}, untouched: {blank.mean():.3f}")
+    assert (same | blank).all()
+    assert blank.mean() > 0.05
+    if engine == 2:  # a cancellable wavefront frame runs in passes of whole pixel-block rows
+        rows_done = same.all(1) | (full == 127).all(-1).all(1)
+        rows_blank = blank.all(1)
+        assert (rows_done | rows_blank).all()
+
+
+def test_raytracer_stop(rt_gpu):
+    """RayTracer::stop() on a frame in flight, through the drop-in class."""
+    import time
+    rt = rt_gpu
+    log("case: RayTracer::start then RayTracer::stop, workload 5 at 2048x1152")
+    hs = rt.HostScene.workload(5, width=2048, height=1152, detail=8, n_spheres=64)
+    t0 = time.perf_counter()
+    full, use_time = hs.raytrace(0x80, max_level=8)
+    log(f"  full frame {time.perf_counter() - t0:.2f} s (useTime {use_time:.3f} s)")
+    out, rc, finished = hs.raytrace_stopped(0x80, 8, use_time / 3)
+    assert not finished, "the frame finished before the stop"
+    assert rc == -4, rc
+    W, H = 2048, 1152
+    a = out[:H * W * 3].reshape(H, W, 3)
+    f = full[:H * W * 3].reshape(H, W, 3)
+    same, blank = (a == f).all(-1), (a == 127).all(-1)
+    log(f"  rendered before the stop: {same.mean():.3f}")
+    assert (same | blank).all() and blank.mean() > 0.05
 
 
 def test_raytracer_drop_in(rt_gpu):

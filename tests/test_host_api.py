@@ -143,6 +143,54 @@ def test_obj_loader_quads_parts_rescale(tmp_path):
     assert d.materials[p1.mtl].ambient.x == np.float32(0.1) and d.materials[p1.mtl].shiness == 20
 
 
+def write_bmp(path, w, h, pixels: bytes, off=54, size_image=0):
+    """A 24-bit BMP: BITMAPFILEHEADER + BITMAPINFOHEADER, pixel bytes at `off`."""
+    import struct
+    fh = b"BM" + struct.pack("<IHHI", off + len(pixels), 0, 0, off)
+    ih = struct.pack("<IiiHHIIiiII", 40, w, h, 1, 24, 0, size_image, 2835, 2835, 0, 0)
+    path.write_bytes(fh + ih + b"\xee" * (off - 54) + pixels)
+
+
+def test_bmp_texture_known_texels(tmp_path, monkeypatch):
+    """BMP reading (Model.cpp:283-312) through an MTL map_Kd: the texture file
+    is the map's base name with extension .bmp, opened from the working
+    directory; w x h from the info header; biSizeImage bytes (w h 3 when 0)
+    read at bfOffBits, kept as stored (B, G, R, bottom row first).  Texel
+    (x, y) is then bytes 3 (y w + x) .. + 2 (3DElement.cpp:443-449)."""
+    rng = np.random.default_rng(5)
+    w, h = 4, 3
+    pix = rng.integers(0, 256, w * h * 3, dtype=np.uint8).tobytes()
+    write_bmp(tmp_path / "wood.bmp", w, h, pix, off=70)  # 16 bytes between header and pixels
+    w2, h2 = 2, 5
+    pix2 = rng.integers(0, 256, w2 * h2 * 3, dtype=np.uint8).tobytes()
+    write_bmp(tmp_path / "tile.bmp", w2, h2, pix2, size_image=w2 * h2 * 3)
+    (tmp_path / "t.mtl").write_text("newmtl a\nKd 1 1 1\nmap_Kd C:\\assets\\wood.png\n"
+                                     "newmtl b\nmap_Kd tile.tga\nnewmtl c\nmap_Kd wood.jpg\n")
+    (tmp_path / "t.obj").write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nvn 0 0 1\nvt 0 0\nvt 1 0\nvt 0 1\n"
+                                     "usemtl a\nf 1/1/1 2/2/1 3/3/1\nusemtl b\nf 1/1/1 2/2/1 3/3/1\n"
+                                     "usemtl c\nf 1/1/1 2/2/1 3/3/1\n")
+    monkeypatch.chdir(tmp_path)
+    hs = rt.HostScene.new()
+    i = hs.add_model_obj("t.obj", "t.mtl")
+    d = desc(hs)
+    o = d.objects[i]
+    assert o.part_count == 3
+    texs = []
+    for k in range(3):
+        m = d.materials[d.parts[o.part_begin + k].mtl]
+        assert m.tex >= 0
+        t = d.textures[m.tex]
+        texs.append((t.w, t.h, bytes(np.ctypeslib.as_array(t.bgr, shape=(t.w * t.h * 3,)))))
+    assert texs[0] == (w, h, pix) and texs[1] == (w2, h2, pix2)
+    assert texs[2] == texs[0]  # wood.jpg -> wood.bmp, already loaded: the same texture
+    # a file that is not a bitmap gives no texture, as in the reference
+    (tmp_path / "bad.bmp").write_bytes(b"XX" + bytes(60))
+    (tmp_path / "bad.mtl").write_text("newmtl a\nmap_Kd bad.bmp\n")
+    j = hs.add_model_obj("t.obj", "bad.mtl")
+    d = desc(hs)
+    assert d.materials[d.parts[d.objects[j].part_begin].mtl].tex == -1
+
+
 def test_missing_obj_is_an_empty_model(tmp_path):
     hs = rt.HostScene.new()
     i = hs.add_model_obj(str(tmp_path / "none.obj"), str(tmp_path / "none.mtl"))

@@ -35,6 +35,12 @@ MUTANTS = {
     8: "plane: parallel threshold compared in float",
     9: "quantisation: rounding instead of truncation",
     10: "closest hit: ties go to the later object",
+    11: "plane texture coordinates: axisx and axisy swapped",
+    12: "plane texture coordinates: the /5 dropped",
+    13: "texel: bytes read as R, G, B instead of B, G, R",
+    14: "box: the x face normal points the wrong way",
+    15: "ballplane: sphere centres visited cy-major (identities reordered)",
+    16: "texel: negative coordinates not wrapped into [0, 1)",
 }
 
 
