@@ -76,6 +76,19 @@ int rtg_host_add_light(rtg_host_scene_t scene, int type, const float comp[3], co
 int rtg_host_move_pos(rtg_host_scene_t scene, int type, int num, const float v[3]);
 /* Scene::Switch (Scene.cpp:282-301). */
 int rtg_host_switch(rtg_host_scene_t scene, int type, int num, int show);
+/* Scene::ChgLightComp (Scene.cpp:157-188): type MY_LIGHT_COMPENT (1) weights
+ * the ambient / diffuse / specular parts by v[0..2] and renormalises them to
+ * the luminance; MY_LIGHT_LUMI (2) scales the luminance by v[3]. */
+int rtg_host_chg_light_comp(rtg_host_scene_t scene, int type, int num, const float v[4]);
+/* Scene::ChgMtl(num, Normal) (Scene.cpp:202-224): recolour the diffuse term;
+ * clr[3] < 0.5 keeps the old r + g + b. */
+int rtg_host_recolour(rtg_host_scene_t scene, int num, const float clr[4]);
+/* Scene::ChgMtl(num, MtlLiby[lib_index]) (Scene.cpp:190-200). */
+int rtg_host_chg_mtl_library(rtg_host_scene_t scene, int num, int lib_index);
+/* Scene::Delete (Scene.cpp:226-246): type MY_MODEL_OBJECT (2) or MY_MODEL_LIGHT (1). */
+int rtg_host_delete(rtg_host_scene_t scene, int type, int num);
+/* Scene::Lights[num] as a flat record. */
+int rtg_host_light(rtg_host_scene_t scene, int num, rtg_light *out);
 /* Scene::ChgMtl(num, Material) (Scene.cpp:190-200), material given as a flat record. */
 int rtg_host_set_material(rtg_host_scene_t scene, int num, const rtg_material *m);
 /* Direct writes the reference's callers make to public members:
@@ -102,6 +115,34 @@ int rtg_host_raytrace(rtg_host_scene_t scene, int type, int max_level, int frame
  * at the stop. */
 int rtg_host_raytrace_stop(rtg_host_scene_t scene, int type, int max_level, double stop_after_s, uint8_t *out,
                            int *finished_before_stop);
+
+/* One ray against one object of the scene: DrawObject::intersect
+ * (3DElement.h:201; Basic3DObject.cpp:135-190, 274-300, 376-410, 486-552;
+ * Model.cpp:748-811), after the object's RTPrepare (Model.cpp:402-480), as the
+ * reference's renderer calls it for each object in turn.  The ray is
+ * Ray(origin, direction, type) with isInside (0 or 0xFF); the hit it must
+ * improve on has distance `prev_distance` (1e20: HitRes()) and identity
+ * `prev_obj` (what a previous call returned in `out->obj`, or 0).  `min` is
+ * the any-hit bound.  `out` receives the returned HitRes; out->obj is the
+ * identity the reference compares (the object's, a triangle copy's, or a
+ * BallPlane sphere's address), out->hit is 1 when the object replaced the
+ * hit, 0 when it returned `prev` unchanged. */
+typedef struct rtg_host_ray {
+    float origin[4], direction[4];
+    int32_t type, inside;
+} rtg_host_ray;
+typedef struct rtg_host_hit {
+    float position[4], normal[4];
+    float u, v, distance, rfr;
+    int64_t obj;      /* HitRes::obj */
+    int64_t object;   /* address of the DrawObject itself, for identities this + k */
+    int32_t inside;   /* HitRes::isInside (0 or 0xFF) */
+    int32_t hit;
+    int32_t has_tex;  /* HitRes::tex != NULL */
+    int32_t pad;
+} rtg_host_hit;
+int rtg_host_intersect(rtg_host_scene_t scene, int num, const rtg_host_ray *ray, int64_t prev_obj,
+                       float prev_distance, float min, rtg_host_hit *out);
 
 /* Write a frame as an 8-bit RGB PNG: what the reference's viewer shows of
  * RayTracer::output (main.cpp:207-208, an RGB texture whose row 0 is the

@@ -129,6 +129,16 @@ class rtg_gen_request(C.Structure):
                 ("arrival_s", C.c_double)]
 
 
+class rtg_host_ray(C.Structure):
+    _fields_ = [("origin", C.c_float * 4), ("direction", C.c_float * 4), ("type", C.c_int32), ("inside", C.c_int32)]
+
+
+class rtg_host_hit(C.Structure):
+    _fields_ = [("position", C.c_float * 4), ("normal", C.c_float * 4), ("u", C.c_float), ("v", C.c_float),
+                ("distance", C.c_float), ("rfr", C.c_float), ("obj", C.c_int64), ("object", C.c_int64),
+                ("inside", C.c_int32), ("hit", C.c_int32), ("has_tex", C.c_int32), ("pad", C.c_int32)]
+
+
 class rtg_kernel_time(C.Structure):
     _fields_ = [("name", C.c_char * 24), ("launches", C.c_uint32), ("pad", C.c_uint32), ("items", C.c_uint64),
                 ("ms", C.c_double)]
@@ -231,6 +241,12 @@ SIGNATURES = {
     "rtg_host_resize": (_I, [_P, _I, _I]),
     "rtg_host_raytrace": (_I, [_P, _I, _I, _I, _P, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "rtg_host_raytrace_stop": (_I, [_P, _I, _I, C.c_double, _P, C.POINTER(C.c_int)]),
+    "rtg_host_intersect": (_I, [_P, _I, C.c_void_p, C.c_int64, C.c_float, C.c_float, C.c_void_p]),
+    "rtg_host_chg_light_comp": (_I, [_P, _I, _I, C.POINTER(C.c_float)]),
+    "rtg_host_recolour": (_I, [_P, _I, C.POINTER(C.c_float)]),
+    "rtg_host_chg_mtl_library": (_I, [_P, _I, _I]),
+    "rtg_host_delete": (_I, [_P, _I, _I]),
+    "rtg_host_light": (_I, [_P, _I, C.c_void_p]),
 }
 
 _lib = None

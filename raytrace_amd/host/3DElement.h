@@ -102,6 +102,17 @@ public:
     void SetMtl(const uint8_t prop, const float val);
 };
 
+// World-space triangle of Model::RTPrepare's octant bins (3DElement.h:122-127):
+// the edges from p0, p0 + position, and the (part, triangle) it copies.  A
+// straddling triangle has one copy per bin; a hit's HitRes::obj is the address
+// of the copy that was hit.
+struct alignas(16) clTri {
+    Vertex axisu, axisv, p0;
+    int16_t numa = 0, numb = 0;
+    clTri(const Vertex &u = Vertex(), const Vertex &v = Vertex(), const Vertex &p = Vertex())
+        : axisu(u), axisv(v), p0(p) {}
+};
+
 class Triangle {
 public:
     Vertex points[3];
@@ -154,9 +165,9 @@ public:
     operator bool();
 };
 
-// Base of every renderable object (3DElement.h:185-202).  The per-object CPU
-// `intersect` of the reference is not part of this build: the render path
-// intersects on the device (librtg); RTPrepare is kept as a no-op hook.
+// Base of every renderable object (3DElement.h:185-202).  The renderer
+// (RayTracer) intersects on the device and needs neither RTPrepare nor
+// intersect; they remain for callers that query objects one ray at a time.
 class DrawObject {
 protected:
     GLuint GLListNum;
@@ -170,8 +181,18 @@ public:
     virtual ~DrawObject() {}
     virtual void SetMtl(const Material &mtl) { this->mtl = mtl; }
     void GLDraw() {}
-    virtual void GLPrepare() {}
+    // Display-list upload of the viewer; headless, every object's is a no-op.
+    virtual void GLPrepare() = 0;
+    // Per-frame preparation for intersect (a Model bins its triangles).
     virtual void RTPrepare() {}
+    // One step of the renderer's object scan (RayTracer.cpp:458-465): the hit
+    // of `ray` on this object if it is closer than hr.distance, else `hr`
+    // unchanged.  hr.obj names the object (or triangle copy, or BallPlane
+    // sphere) the ray leaves, which is skipped or, from inside, exited.
+    // `min`: a Model stops at the first triangle closer than it (the any-hit
+    // bound of shadow rays).  Runs on the host with the primitive tests the
+    // device kernels use.
+    virtual HitRes intersect(const Ray &ray, const HitRes &hr, const float min = 0) = 0;
 };
 
 class Light {

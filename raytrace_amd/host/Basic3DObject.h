@@ -1,6 +1,6 @@
 // Basic3DObject.h — headless Sphere / Box / Plane / BallPlane of the drop-in API
-// (reference Basic3DObject.h:5-68).  Only the scene-side state lives here; ray
-// intersection of these primitives runs on the device (librtg).
+// (reference Basic3DObject.h:5-68).  The renderer intersects them on the
+// device (librtg); intersect() answers one ray on the host (intersect.cpp).
 #pragma once
 #include "3DElement.h"
 
@@ -14,6 +14,8 @@ public:
     Sphere(const float r = 1.0f, GLuint lnum = 0);
     ~Sphere() override {}
     float getRadius() const { return radius; }
+    void GLPrepare() override {}
+    HitRes intersect(const Ray &ray, const HitRes &hr, const float min = 0) override;
 };
 
 class Box : public DrawObject {
@@ -24,6 +26,8 @@ private:
 public:
     Box(const float len = 2.0, GLuint lnum = 0);
     Box(const float l, const float w, const float h, GLuint lnum = 0);
+    void GLPrepare() override {}
+    HitRes intersect(const Ray &ray, const HitRes &hr, const float dmin = 0) override;
 };
 
 class Plane : public DrawObject {
@@ -40,6 +44,8 @@ public:
     void setTex(const Texture &tex) { this->tex = tex; }
     const Normal &getAxisX() const { return axisx; }
     const Normal &getAxisY() const { return axisy; }
+    void GLPrepare() override {}
+    HitRes intersect(const Ray &ray, const HitRes &hr, const float min = 0) override;
 };
 
 class BallPlane : public DrawObject {
@@ -53,4 +59,6 @@ public:
     BallPlane(const float r = 0.3f, GLuint lnum = 0);
     ~BallPlane() override {}
     void rotate(const Vertex &v);
+    void GLPrepare() override {}
+    HitRes intersect(const Ray &ray, const HitRes &hr, const float min = 0) override;
 };

@@ -83,7 +83,9 @@ void Model::reset() {
     texs.clear();
     mtls.clear();
     parts.clear();
+    octclparts.clear();
     borders.clear();
+    bboxs.clear();
     vers.clear();
     nors.clear();
     txcs.clear();

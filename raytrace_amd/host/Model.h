@@ -1,8 +1,8 @@
 // Model.h — headless triangle-mesh object of the drop-in API (reference
 // Model.h:6-50).  Holds the same per-part triangle lists, border boxes and
-// material tables as the reference; the per-frame octant binning of
-// Model::RTPrepare (Model.cpp:402-480) is replaced by the device BVH built in
-// librtg, so RTPrepare is a no-op here.
+// material tables as the reference.  The renderer builds a BVH on the device
+// (librtg) and does not call RTPrepare; RTPrepare's octant bins
+// (Model.cpp:402-480) serve intersect() on the host (intersect.cpp).
 #pragma once
 #include "3DElement.h"
 
@@ -28,7 +28,9 @@ public:
     vector<Normal> nors;
     vector<Coord2D> txcs;
     vector<vector<Triangle>> parts;
+    vector<vector<clTri>> octclparts;  // RTPrepare: 8 bins per part, each ending in a sentinel
     vector<Vertex> borders;
+    vector<Vertex> bboxs;              // RTPrepare: part border boxes + position
     vector<Material> mtls;
     vector<Texture> texs;
     vector<int8_t> part_mtl, mtl_tex;
@@ -50,6 +52,9 @@ public:
     int32_t loadMesh(const vector<MeshPart> &mesh, const vector<Material> &materials, bool rescaleTo8 = true);
     void zRotate();
     virtual void SetMtl(const Material &mtl) override;
+    void GLPrepare() override {}
+    void RTPrepare() override;
+    HitRes intersect(const Ray &ray, const HitRes &hr, const float min = 0) override;
     const Vertex &getVerMin() const { return VerMin; }
     const Vertex &getVerMax() const { return VerMax; }
 };

@@ -41,4 +41,7 @@ public:
     bool Switch(const uint8_t type, const uint8_t num, const bool isShow);
     void DrawScene() {}
     void DrawLight() {}
+
+private:
+    uint8_t adopt(DrawObject *obj);  // append to Objects, return its index
 };

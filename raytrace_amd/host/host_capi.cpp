@@ -103,6 +103,33 @@ int rtg_host_switch(rtg_host_scene_t h, int type, int num, int show) {
     if (!valid(h)) return RTG_E_INVALID;
     return h->scene->Switch((uint8_t)type, (uint8_t)num, show != 0) ? 1 : 0;
 }
+int rtg_host_chg_light_comp(rtg_host_scene_t h, int type, int num, const float v[4]) {
+    if (!valid(h) || v == nullptr) return RTG_E_INVALID;
+    return h->scene->ChgLightComp((uint8_t)type, (uint8_t)num, Vertex(v[0], v[1], v[2], v[3])) ? RTG_OK
+                                                                                             : RTG_E_INVALID;
+}
+int rtg_host_recolour(rtg_host_scene_t h, int num, const float clr[4]) {
+    if (!valid(h) || clr == nullptr) return RTG_E_INVALID;
+    return h->scene->ChgMtl((uint8_t)num, Normal(clr[0], clr[1], clr[2], clr[3])) ? RTG_OK : RTG_E_INVALID;
+}
+int rtg_host_chg_mtl_library(rtg_host_scene_t h, int num, int lib_index) {
+    if (!valid(h) || lib_index < 0 || lib_index >= (int)h->scene->MtlLiby.size()) return RTG_E_INVALID;
+    return h->scene->ChgMtl((uint8_t)num, h->scene->MtlLiby[lib_index]) ? RTG_OK : RTG_E_INVALID;
+}
+int rtg_host_delete(rtg_host_scene_t h, int type, int num) {
+    if (!valid(h)) return RTG_E_INVALID;
+    return h->scene->Delete((uint8_t)type, (uint8_t)num) ? RTG_OK : RTG_E_INVALID;
+}
+int rtg_host_light(rtg_host_scene_t h, int num, rtg_light *out) {
+    if (!valid(h) || out == nullptr || num < 0 || num >= (int)h->scene->Lights.size()) return RTG_E_INVALID;
+    const Light &l = h->scene->Lights[num];
+    auto v4 = [](const Vertex &v) { return rtg_vec4{v.x, v.y, v.z, v.w}; };
+    *out = rtg_light{};
+    out->type = l.type, out->on = l.bLight;
+    out->position = v4(l.position), out->ambient = v4(l.ambient), out->diffuse = v4(l.diffuse);
+    out->specular = v4(l.specular), out->attenuation = v4(l.attenuation);
+    return RTG_OK;
+}
 int rtg_host_set_material(rtg_host_scene_t h, int num, const rtg_material *m) {
     if (!valid(h) || m == nullptr) return RTG_E_INVALID;
     Material mt;
@@ -151,6 +178,32 @@ int rtg_host_raytrace(rtg_host_scene_t h, int type, int max_level, int frames, u
     if (use_time != nullptr) *use_time = rt.useTime;
     if (uploads != nullptr) *uploads = rt.uploads;
     return rt.lastStatus;
+}
+
+int rtg_host_intersect(rtg_host_scene_t h, int num, const rtg_host_ray *ray, int64_t prev_obj, float prev_distance,
+                       float min, rtg_host_hit *out) {
+    if (!valid(h) || ray == nullptr || out == nullptr || num < 0 || num >= (int)h->scene->Objects.size())
+        return RTG_E_INVALID;
+    DrawObject *obj = h->scene->Objects[num];
+    obj->RTPrepare();
+    auto vtx = [](const float *a) { return Vertex(a[0], a[1], a[2], a[3]); };
+    const float *d = ray->direction;
+    Ray r(vtx(ray->origin), Normal(d[0], d[1], d[2], d[3]), (uint8_t)ray->type);  // taken as given, not renormalised
+    r.isInside = (uint8_t)ray->inside;
+    HitRes prev(prev_distance);
+    prev.obj = (intptr_t)prev_obj;
+    const HitRes res = obj->intersect(r, prev, min);
+    memset(out, 0, sizeof(*out));
+    const float *p = &res.position.x, *n = &res.normal.x;
+    for (int k = 0; k < 4; ++k) out->position[k] = p[k], out->normal[k] = n[k];
+    out->u = res.tcoord.u, out->v = res.tcoord.v;
+    out->distance = res.distance, out->rfr = res.rfr;
+    out->obj = (int64_t)res.obj;
+    out->object = (int64_t)(intptr_t)obj;
+    out->inside = res.isInside;
+    out->hit = !(res.obj == prev.obj && res.distance == prev.distance);
+    out->has_tex = res.tex != nullptr;
+    return RTG_OK;
 }
 
 int rtg_host_raytrace_stop(rtg_host_scene_t h, int type, int max_level, double stop_after_s, uint8_t *out,

@@ -252,3 +252,62 @@ def test_reused_output_buffers_are_checked():
         rt.render_batch([(_NoScene(), p)], out=[{"rgb": good["rgb"], "ids": np.empty((32, 64, 2), np.int64)}])
     with pytest.raises(ValueError):
         rt.render_batch([(_NoScene(), p)], ids=False, out=[{"rgb": np.empty((32, 128, 3), np.uint8)[:, ::2], "ids": None}])
+
+
+def _light(hs, k):
+    out = _capi.rtg_light()
+    _capi.check(_capi.load().rtg_host_light(hs.h, k, C.byref(out)), "rtg_host_light")
+    return out
+
+
+def test_scene_mutators_closed_forms():
+    """Scene edits (Scene.cpp:157-301) against what they must produce:
+    ChgLightComp keeps every channel's ambient + diffuse + specular at the
+    luminance while weighting the parts; LUMI scales all parts; ChgMtl(colour)
+    keeps r + g + b (w < 0.5) or replaces the colour; the 'wall' material
+    clears a plane's checker; Delete removes exactly one entry; the ninth
+    light is refused."""
+    L = _capi.load()
+    hs = rt.HostScene.new()
+    hs.add_light(2, (0.2, 0.5, 0.3), (1, 0, 0, 2.0))
+    a = _light(hs, 0)
+    for c in "xyz":  # AddLight: the parts sum to the luminance
+        s = getattr(a.ambient, c) + getattr(a.diffuse, c) + getattr(a.specular, c)
+        assert abs(s - 2.0) < 1e-6
+    assert L.rtg_host_chg_light_comp(hs.h, 1, 0, _capi.floats(1.0, 2.0, 0.5, 0.0)) == 0
+    b = _light(hs, 0)
+    for c in "xyz":
+        pa, pd, ps = (getattr(a.ambient, c), getattr(a.diffuse, c) * 2.0, getattr(a.specular, c) * 0.5)
+        tot = pa + pd + ps
+        want = np.array([pa, pd, ps]) / tot * 2.0
+        got = np.array([getattr(b.ambient, c), getattr(b.diffuse, c), getattr(b.specular, c)])
+        assert np.allclose(got, want, rtol=1e-6), (c, got, want)
+    assert L.rtg_host_chg_light_comp(hs.h, 2, 0, _capi.floats(0, 0, 0, 1.5)) == 0
+    c_ = _light(hs, 0)
+    assert abs(c_.attenuation.w - 3.0) < 1e-6 and abs(c_.diffuse.y - b.diffuse.y * 1.5) < 1e-6
+    assert L.rtg_host_chg_light_comp(hs.h, 3, 0, _capi.floats(1, 1, 1, 1)) != 0  # unknown type
+    assert L.rtg_host_chg_light_comp(hs.h, 1, 5, _capi.floats(1, 1, 1, 1)) != 0  # no such light
+    for k in range(7):
+        assert hs.add_light(2, (1, 1, 1), (1, 0, 0, 1)) == k + 1
+    assert hs.add_light(2, (1, 1, 1), (1, 0, 0, 1)) == 255
+    # recolour
+    s = hs.add_sphere(1.0)  # bas-sphere diffuse (0.1, 0.5, 0.8)
+    assert L.rtg_host_recolour(hs.h, s, _capi.floats(1.0, 0.0, 0.0, 0.0)) == 0
+    m = desc(hs).materials[desc(hs).objects[s].mtl]
+    assert abs(m.diffuse.x - 1.4) < 1e-6 and m.diffuse.y == 0 and m.diffuse.z == 0
+    assert L.rtg_host_recolour(hs.h, s, _capi.floats(0.25, 0.5, 0.75, 1.0)) == 0
+    m = desc(hs).materials[desc(hs).objects[s].mtl]
+    assert (m.diffuse.x, m.diffuse.y, m.diffuse.z) == (0.25, 0.5, 0.75)
+    # the wall material clears a plane's checker; other objects keep theirs
+    p = hs.add_plane()
+    t0 = desc(hs).textures[desc(hs).objects[p].tex]
+    assert len(set(np.ctypeslib.as_array(t0.bgr, shape=(48,)).tolist())) == 2
+    assert L.rtg_host_chg_mtl_library(hs.h, p, 5) == 0
+    t1 = desc(hs).textures[desc(hs).objects[p].tex]
+    assert set(np.ctypeslib.as_array(t1.bgr, shape=(48,)).tolist()) == {255}
+    # delete
+    n = desc(hs).n_objects
+    assert L.rtg_host_delete(hs.h, 2, 0) == 0 and desc(hs).n_objects == n - 1
+    assert desc(hs).objects[0].type == 4  # the plane moved down
+    assert L.rtg_host_delete(hs.h, 2, 9) != 0
+    assert L.rtg_host_delete(hs.h, 1, 0) == 0
