@@ -256,6 +256,19 @@ int rtg_render(rtg_scene_t scene, const rtg_params *params, uint8_t *rgb,
 int rtg_render_device(rtg_scene_t scene, const rtg_params *params, uint8_t *rgb_dev,
                       int32_t *ids_dev, float *tprim_dev, uint64_t *stats_dev, void *stream);
 
+/* rtg_render_device that allocates nothing: no counterpart in the reference.
+ * For callers that cannot allocate device memory while they render (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results as rtg_render_device with the
+ * megakernel engine (engine AUTO or MEGAKERNEL; no FP64 re-test, no kernel
+ * timing): every scratch buffer lives in one fixed device array of the
+ * library (48.1 MiB per GPU, held until the process ends), set up by rtg_init.
+ * Frames up to RTG_MAX_DIM x RTG_MAX_DIM; larger sizes return RTG_E_LIMIT.
+ * Calls on one GPU share that array: calls that overlap (two streams, two
+ * threads) race.  `stats_dev` NULL: the ray counts stay in the scratch. */
+int rtg_render_device_noalloc(rtg_scene_t scene, const rtg_params *params, uint8_t *rgb_dev, int32_t *ids_dev,
+                              float *tprim_dev, uint64_t *stats_dev, void *stream);
+
 /* Non-blocking render: RayTracer::start's detached workers + monitor
  * (RayTracer.cpp:664-695).  rtg_poll returns RTG_E_PENDING while running, RTG_OK
  * when finished (isFinish, RayTracer.h:47), RTG_E_CANCELLED after rtg_cancel.

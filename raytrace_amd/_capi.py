@@ -207,6 +207,7 @@ SIGNATURES = {
     "rtg_scene_build_ms": (_I, [_P, C.POINTER(C.c_double)]),
     "rtg_render": (_I, [_P, C.POINTER(rtg_params), _P, _P, _P, C.POINTER(rtg_stats)]),
     "rtg_render_device": (_I, [_P, C.POINTER(rtg_params), _P, _P, _P, _P, _P]),
+    "rtg_render_device_noalloc": (_I, [_P, C.POINTER(rtg_params), _P, _P, _P, _P, _P]),
     "rtg_render_async": (_I, [_P, C.POINTER(rtg_params), _P, _P, _P, C.POINTER(_P)]),
     "rtg_poll": (_I, [_P]),
     "rtg_cancel": (_I, [_P]),

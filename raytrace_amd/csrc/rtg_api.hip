@@ -273,6 +273,8 @@ int rtg_init(int device) {
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(RTG_E_DEVICE, "no HIP device available");
     if (device < 0 || device >= n) return fail(RTG_E_INVALID, "device index out of range");
     RTG_HIP(hipSetDevice(device));
+    int st = noalloc_setup(device);
+    if (st != RTG_OK) return st;
     std::unique_ptr<Arena> A(new Arena());
     A->device = device;
     hipDeviceProp_t prop;

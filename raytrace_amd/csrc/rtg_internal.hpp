@@ -220,6 +220,24 @@ struct Cancel {
 // may be null.
 int launch_megakernel(const SceneRes &S, const rtg_params &p, int W, int H, const RenderOut &o, hipStream_t stream,
                       Arena &A, const Cancel *cancel);
+// The megakernel's scratch: ray-generation tables (W + H floats, written for
+// the frame's W, H and `dp`), the traversal stacks' spill area for `blocks`
+// persistent blocks (spill_ints(blocks) ints).
+struct MegaScratch {
+    float *tables;
+    int *spill;
+    int blocks;
+    double dp;
+};
+// Enqueue the megakernel with the given scratch (tables already written);
+// allocates nothing.
+int enqueue_megakernel(const SceneRes &S, const rtg_params &p, int W, int H, const RenderOut &o, hipStream_t stream,
+                       Arena &A, const Cancel *cancel, const MegaScratch &ms);
+// Enqueue the ray-generation tables (float)((x - W/2) dp), (float)((y - H/2) dp)
+// into W + H floats at `tab` on `stream`; allocates nothing.
+int write_raygen_tables(float *tab, int W, int H, double dp, hipStream_t stream);
+// Set up the no-allocation entry point's scratch on `device` (current).
+int noalloc_setup(int device);
 // Launch one frame of the wavefront engine on `stream` (asynchronous) with
 // the buffers of `ws`; `cancel` may be null.
 // Host copies of device results staged in pinned memory, run on up to 8

@@ -116,6 +116,12 @@ __global__ void k_raygen_tables(float *tab, int W, int H, double dp) {
         tab[i] = i < W ? (float)((double)(i - W / 2) * dp) : (float)((double)(i - W - H / 2) * dp);
 }
 
+int write_raygen_tables(float *tab, int W, int H, double dp, hipStream_t stream) {
+    k_raygen_tables<<<(W + H + 255) / 256, 256, 0, stream>>>(tab, W, H, dp);
+    RTG_HIP(hipGetLastError());
+    return RTG_OK;
+}
+
 int upload_raygen_tables(const SceneRes &S, int W, int H, hipStream_t stream, Workspace &ws, double &dp) {
     dp = tan((double)S.camera.fovy * 3.1415926535897932384 / 360) / (H / 2);
     const uint32_t fb = __builtin_bit_cast(uint32_t, S.camera.fovy);
@@ -135,31 +141,38 @@ int upload_raygen_tables(const SceneRes &S, int W, int H, hipStream_t stream, Wo
 
 int persistent_blocks(const Arena &A) { return std::max(8, (A.cus > 0 ? A.cus : 256) * 8); }
 
-int launch_megakernel(const SceneRes &S, const rtg_params &p, int W, int H, const RenderOut &o, hipStream_t stream,
-                      Arena &A, const Cancel *cancel) {
-    double dp;
-    int st = upload_raygen_tables(S, W, H, stream, A.ws, dp);
-    if (st != RTG_OK) return st;
-    const int blocks = persistent_blocks(A);
-    if (A.ws.grow(A.ws.spill, spill_ints(blocks) * sizeof(int))) return fail(RTG_E_NOMEM, "spill");
+int enqueue_megakernel(const SceneRes &S, const rtg_params &p, int W, int H, const RenderOut &o, hipStream_t stream,
+                       Arena &A, const Cancel *cancel, const MegaScratch &ms) {
     MegaArgs a;
     a.W = W, a.H = H, a.spp = p.spp > 0 ? p.spp : 1, a.max_level = p.max_level, a.tile_compat = p.tile_compat;
     a.mode = p.mode;
     a.bw = (W + 15) / 16;
     a.nblocks = a.bw * ((H + 7) / 8);
-    a.dpf = (float)dp;
+    a.dpf = (float)ms.dp;
     a.seed = p.seed;
-    a.ax = A.ws.tables.as<float>();
-    a.by = A.ws.tables.as<float>() + W;
+    a.ax = ms.tables;
+    a.by = ms.tables + W;
     a.rgb = o.rgb, a.ids = o.ids, a.tprim = o.tprim;
     a.counts = o.counts;
     a.cancel = cancel ? cancel->dev : nullptr;
-    a.spill = A.ws.spill.as<int>();
+    a.spill = ms.spill;
     const int kt = ktime_begin(A, stream, KT_MEGA, (uint64_t)W * H * a.spp);
-    k_megakernel<<<blocks, kBlockThreads, 0, stream>>>(S.view, a);
+    k_megakernel<<<ms.blocks, kBlockThreads, 0, stream>>>(S.view, a);
     ktime_end(A, stream, kt);
     RTG_HIP(hipGetLastError());
     return RTG_OK;
+}
+
+int launch_megakernel(const SceneRes &S, const rtg_params &p, int W, int H, const RenderOut &o, hipStream_t stream,
+                      Arena &A, const Cancel *cancel) {
+    MegaScratch ms{};
+    int st = upload_raygen_tables(S, W, H, stream, A.ws, ms.dp);
+    if (st != RTG_OK) return st;
+    ms.blocks = persistent_blocks(A);
+    if (A.ws.grow(A.ws.spill, spill_ints(ms.blocks) * sizeof(int))) return fail(RTG_E_NOMEM, "spill");
+    ms.tables = A.ws.tables.as<float>();
+    ms.spill = A.ws.spill.as<int>();
+    return enqueue_megakernel(S, p, W, H, o, stream, A, cancel, ms);
 }
 
 }  // namespace rtg
