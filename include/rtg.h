@@ -262,7 +262,7 @@ int rtg_render_device(rtg_scene_t scene, const rtg_params *params, uint8_t *rgb_
  * hipMalloc).  Same arguments and results as rtg_render_device with the
  * megakernel engine (engine AUTO or MEGAKERNEL; no FP64 re-test, no kernel
  * timing): every scratch buffer lives in one fixed device array of the
- * library (48.1 MiB per GPU, held until the process ends), set up by rtg_init.
+ * library (80.1 MiB per GPU, held until the process ends), set up by rtg_init.
  * Frames up to RTG_MAX_DIM x RTG_MAX_DIM; larger sizes return RTG_E_LIMIT.
  * Calls on one GPU share that array: calls that overlap (two streams, two
  * threads) race.  `stats_dev` NULL: the ray counts stay in the scratch. */

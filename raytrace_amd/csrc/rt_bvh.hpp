@@ -13,7 +13,11 @@
 //   4. karras_node     binary radix tree (Karras 2012) over the sorted keys.
 //   5. refit           bottom-up box union (exact: min/max).
 //   6. emit            subtrees of <= kLeafMax primitives become leaves; the
-//                      rest are written as 64-byte BvhNodes.
+//                      rest are written as 64-byte binary BvhNodes.
+//   7. widen           every binary node at even depth becomes a 4-wide node
+//                      (Bvh4Node) whose children are its children, each
+//                      internal one replaced by its own two children; child
+//                      boxes are quantised outward to 8 bits per bound.
 #pragma once
 #include "rt_scene.hpp"
 #include "../../include/rtg.h"
@@ -223,6 +227,112 @@ RTG_HD int top_first(const int *left, const int *right, const int *first, const 
     }
     for (int j = 0; j < nq; ++j) newidx[q[j]] = j;
     return nq;
+}
+
+// ---- 4-wide nodes ---------------------------------------------------------
+
+// The binary node's child `side` (0, 1): its box and link.
+RTG_HD void node2_child(const BvhNode &n, int side, Box6 &b, int &link) {
+    const float4 xy = side ? n.xy1 : n.xy0;
+    b.lo[0] = xy.x, b.hi[0] = xy.y, b.lo[1] = xy.z, b.hi[1] = xy.w;
+    b.lo[2] = side ? n.z01.z : n.z01.x, b.hi[2] = side ? n.z01.w : n.z01.y;
+    link = side ? n.link.y : n.link.x;
+}
+
+// Children of the 4-wide node made from binary node i: each leaf child kept,
+// each internal child replaced by its two children.  Links are in binary
+// terms (leaf codes, or binary node indices at even depth).  Returns 2..4.
+RTG_HD int node4_children(const BvhNode *nodes2, int i, Box6 box[4], int link[4]) {
+    int c = 0;
+    for (int side = 0; side < 2; ++side) {
+        Box6 b;
+        int l;
+        node2_child(nodes2[i], side, b, l);
+        if (l < 0) {
+            box[c] = b, link[c] = l, ++c;
+        } else {
+            for (int s2 = 0; s2 < 2; ++s2) {
+                node2_child(nodes2[l], s2, box[c], link[c]);
+                ++c;
+            }
+        }
+    }
+    return c;
+}
+
+// Depth parity of binary node i (root 0 at depth 0) by its parent links.
+RTG_HD bool node2_even(const int *parent2, int i) {
+    int d = 0;
+    while (i > 0 && d < 256) i = parent2[i], ++d;
+    return (d & 1) == 0;
+}
+
+// Byte bounds of `nc` child boxes on one axis: the smallest power-of-two step
+// s = 2^(e - 127) for which every child's [lo, hi] is covered by
+// [org + qlo s, org + qhi s] with org the union's lower bound and q in 0..255,
+// the covering checked with the decode's own float arithmetic.  An axis no
+// step covers (infinite bounds) gets org = -FLT_MAX, s = 2^127 and the full
+// byte range, which decodes to [-FLT_MAX, +inf].
+RTG_HD void quantize_axis(const Box6 *b, int nc, int a, float &org, int &e, uint8_t *qlo, uint8_t *qhi) {
+    float lo = b[0].lo[a], hi = b[0].hi[a];
+    for (int c = 1; c < nc; ++c) lo = fminf(lo, b[c].lo[a]), hi = fmaxf(hi, b[c].hi[a]);
+    const double ext = (double)hi - (double)lo;
+    int k = -126;  // step 2^k
+    if (ext > 0.0 && ext < 1e300) {
+        int x;
+        const double m = frexp(ext / 255.0, &x);  // ext / 255 = m 2^x, m in [0.5, 1)
+        k = m == 0.5 ? x - 1 : x;
+        k = k < -126 ? -126 : k;
+    }
+    for (; k <= 127; ++k) {
+        const float s = pow2f(k + 127);
+        bool ok = true;
+        for (int c = 0; c < nc && ok; ++c) {
+            const double fl = floor(((double)b[c].lo[a] - (double)lo) / (double)s);
+            const double fh = ceil(((double)b[c].hi[a] - (double)lo) / (double)s);
+            int ql = fl < 0 ? 0 : (fl > 255 ? 255 : (int)fl);
+            int qh = fh < 0 ? 0 : (fh > 255 ? 255 : (int)fh);
+            while (ql > 0 && !(dequant(lo, (uint32_t)ql, s) <= b[c].lo[a])) --ql;
+            while (qh < 255 && !(dequant(lo, (uint32_t)qh, s) >= b[c].hi[a])) ++qh;
+            ok = dequant(lo, (uint32_t)ql, s) <= b[c].lo[a] && dequant(lo, (uint32_t)qh, s) >= b[c].hi[a];
+            qlo[c] = (uint8_t)ql, qhi[c] = (uint8_t)qh;
+        }
+        if (ok) {
+            org = lo, e = k + 127;
+            return;
+        }
+    }
+    org = -3.40282347e38f, e = 254;
+    for (int c = 0; c < nc; ++c) qlo[c] = 0, qhi[c] = 255;
+}
+
+// The 4-wide node over `nc` (2..4) children with final links `link`.
+RTG_HD Bvh4Node make_node4(const Box6 *box, const int *link, int nc) {
+    Bvh4Node n;
+    uint8_t ql[3][4] = {}, qh[3][4] = {};
+    float org[3];
+    int e[3];
+    for (int a = 0; a < 3; ++a) quantize_axis(box, nc, a, org[a], e[a], ql[a], qh[a]);
+    auto pack = [](const uint8_t *q) {
+        return (uint32_t)q[0] | (uint32_t)q[1] << 8 | (uint32_t)q[2] << 16 | (uint32_t)q[3] << 24;
+    };
+    const uint32_t meta = (uint32_t)e[0] | (uint32_t)e[1] << 8 | (uint32_t)e[2] << 16 | (uint32_t)nc << 24;
+    n.org = make_float4(org[0], org[1], org[2], __builtin_bit_cast(float, meta));
+    n.qxy = make_uint4(pack(ql[0]), pack(qh[0]), pack(ql[1]), pack(qh[1]));
+    n.qz = make_uint4(pack(ql[2]), pack(qh[2]), 0u, 0u);
+    n.link = make_int4(nc > 0 ? link[0] : 0, nc > 1 ? link[1] : 0, nc > 2 ? link[2] : 0, nc > 3 ? link[3] : 0);
+    return n;
+}
+
+// The 4-wide node made from binary node i (at even depth); idx4 maps binary
+// node indices at even depth to 4-wide indices.
+RTG_HD Bvh4Node widen_node(const BvhNode *nodes2, const int *idx4, int i) {
+    Box6 box[4];
+    int link[4];
+    const int nc = node4_children(nodes2, i, box, link);
+    for (int c = 0; c < nc; ++c)
+        if (link[c] >= 0) link[c] = idx4[link[c]];
+    return make_node4(box, link, nc);
 }
 
 RTG_HD BvhNode make_node(const Box6 &b0, const Box6 &b1, int l0, int l1) {

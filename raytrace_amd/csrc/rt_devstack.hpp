@@ -9,7 +9,9 @@ namespace rtg {
 
 constexpr int kBlockThreads = 128;  // two wave64s
 constexpr int kLdsStack = 16;       // entries per thread in LDS (8 KB per block)
-constexpr int kStackMax = 64;       // a Karras tree over 62-bit unique keys is < 63 levels deep
+// A Karras tree over 62-bit unique keys is fewer than 64 binary levels deep,
+// so its 4-wide form has at most 32 levels, each leaving at most 3 entries.
+constexpr int kStackMax = 96;
 
 // Per-thread stack: the first kLdsStack entries in LDS (thread-interleaved, no
 // bank conflicts), the rest in a per-thread global spill area.
@@ -18,7 +20,7 @@ struct DevStack {
     int *spill;
     int gstride;
     int sp;
-    const BvhNode *top;  // LDS copy of the nodes below kTopNodes (one-scene kernels), or null
+    const Bvh4Node *top;  // LDS copy of the nodes below kTopNodes (one-scene kernels), or null
     __device__ void reset() { sp = 0; }
     // The LDS part is addressed as LDS (ds_read / ds_write); the compiler
     // barriers keep each LDS access from being merged with the spill access
@@ -54,11 +56,11 @@ struct DevStack {
     __device__ void on_node() {}
     __device__ void on_prim() {}
     __device__ bool all_lanes(bool v) const { return __all(v); }  // over the wave's active lanes
-    __device__ BvhNode node(const SceneView &S, int i) const {
+    __device__ Bvh4Node node(const SceneView &S, int i) const {
 #if defined(__HIP_DEVICE_COMPILE__)
-        typedef const BvhNode __attribute__((address_space(3))) LdsNode;  // `top` points into LDS
+        typedef const Bvh4Node __attribute__((address_space(3))) LdsNode;  // `top` points into LDS
         if (top != nullptr && i < kTopNodes) {
-            const BvhNode n = ((LdsNode *)top)[i];
+            const Bvh4Node n = ((LdsNode *)top)[i];
             // a compiler barrier (no instruction): keeps this LDS read from
             // being merged with the HBM read below into one flat load
             __asm__ volatile("" ::: "memory");
@@ -72,7 +74,7 @@ struct DevStack {
 // Copy the scene's top nodes into `dst` (a kTopNodes array in LDS) with the
 // whole block, then let `st` read them from there.  Every thread of the
 // block must call it (it synchronises the block).
-__device__ inline void stack_top_nodes(DevStack &st, BvhNode *dst, const SceneView &S) {
+__device__ inline void stack_top_nodes(DevStack &st, Bvh4Node *dst, const SceneView &S) {
     const int nt = min(kTopNodes, S.n_nodes);
     const float4 *src = reinterpret_cast<const float4 *>(S.nodes);
     float4 *d = reinterpret_cast<float4 *>(dst);

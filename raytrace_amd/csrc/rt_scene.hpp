@@ -1,10 +1,12 @@
 // rt_scene.hpp — device-resident scene layout (HBM) and its view.
 //
 // Layout, per scene:
-//   nodes    BVH2, 64 B per node: both children's boxes + links, read as four
-//            16-byte loads.  Children are internal node indices (>= 0) or
-//            leaves ~(first << 3 | (count - 1)) into `prims`.  The nodes of
-//            the top five BFS levels come first (indices < kTopNodes).
+//   nodes    4-wide BVH, 64 B per node (Bvh4Node), read as four 16-byte
+//            loads: up to four children, their boxes quantised to 8 bits per
+//            bound on a power-of-two grid anchored at the node's own box, and
+//            their links.  Children are internal node indices (>= 0) or leaves
+//            ~(first << 3 | (count - 1)) into `prims`.  The top levels come
+//            first (indices < kTopNodes).
 //   prims    48 B per primitive in leaf order (3 x float4):
 //              triangle: (p0w, gtri) (e1, obj) (e2, 0) — the reference's
 //                        clTri values p0 + position, p1 - p0, p2 - p0
@@ -19,6 +21,8 @@
 
 namespace rtg {
 
+// Binary node of the build (both children's boxes and links); the renderer
+// walks the 4-wide nodes made from it.
 struct alignas(16) BvhNode {
     float4 xy0;  // child 0: lo.x, hi.x, lo.y, hi.y
     float4 xy1;  // child 1: lo.x, hi.x, lo.y, hi.y
@@ -27,14 +31,38 @@ struct alignas(16) BvhNode {
 };
 static_assert(sizeof(BvhNode) == 64, "64-byte node");
 
-// The nodes of the top five BFS levels are emitted first (indices <
-// kTopNodes): traversal kernels that render one scene keep them in LDS.
+// 4-wide node.  Child k's box on axis a is [org_a + qlo * s_a, org_a + qhi * s_a]
+// (one float product, exact, and one rounded sum), s_a = 2^(e_a - 127); the
+// bytes are chosen so that the decoded box contains the child's exact box.
+struct alignas(16) Bvh4Node {
+    float4 org;  // x, y, z: origin (the node box's lower corner); w: bits e_x | e_y << 8 | e_z << 16 | count << 24
+    uint4 qxy;   // byte k of x: qlo.x of child k; y: qhi.x; z: qlo.y; w: qhi.y
+    uint4 qz;    // byte k of x: qlo.z; y: qhi.z; z, w: 0
+    int4 link;   // child k's link (slots >= count: 0)
+};
+static_assert(sizeof(Bvh4Node) == 64, "64-byte node");
+
+// The first nodes are the top levels of the tree (indices < kTopNodes):
+// traversal kernels that render one scene keep them in LDS.
 constexpr int kTopNodes = 31;
 
 RTG_HD bool is_leaf(int c) { return c < 0; }
 RTG_HD int leaf_first(int c) { return (~c) >> 3; }
 RTG_HD int leaf_count(int c) { return ((~c) & 7) + 1; }
 RTG_HD int make_leaf(int first, int count) { return ~((first << 3) | (count - 1)); }
+
+RTG_HD int node4_count(const Bvh4Node &n) { return (int)(__builtin_bit_cast(uint32_t, n.org.w) >> 24); }
+RTG_HD float pow2f(int e) { return __builtin_bit_cast(float, (uint32_t)e << 23); }  // 2^(e - 127), e in 1..254
+RTG_HD float dequant(float org, uint32_t q, float s) { return org + (float)q * s; }
+// Child k's decoded box: lo/hi per axis.
+RTG_HD void node4_box(const Bvh4Node &n, int k, float &lx, float &hx, float &ly, float &hy, float &lz, float &hz) {
+    const uint32_t e = __builtin_bit_cast(uint32_t, n.org.w);
+    const float sx = pow2f(e & 0xFF), sy = pow2f((e >> 8) & 0xFF), sz = pow2f((e >> 16) & 0xFF);
+    const int sh = 8 * k;
+    lx = dequant(n.org.x, (n.qxy.x >> sh) & 0xFF, sx), hx = dequant(n.org.x, (n.qxy.y >> sh) & 0xFF, sx);
+    ly = dequant(n.org.y, (n.qxy.z >> sh) & 0xFF, sy), hy = dequant(n.org.y, (n.qxy.w >> sh) & 0xFF, sy);
+    lz = dequant(n.org.z, (n.qz.x >> sh) & 0xFF, sz), hz = dequant(n.org.z, (n.qz.y >> sh) & 0xFF, sz);
+}
 
 struct DevObject {
     int type, show, mtl, tex;
@@ -80,8 +108,8 @@ struct DevCamera {
 };
 
 struct SceneView {
-    const BvhNode *nodes;
-    int n_nodes;  // emitted BVH nodes
+    const Bvh4Node *nodes;
+    int n_nodes;  // 4-wide nodes
     const float4 *prims;
     int root;  // child encoding of the root; 0 with n_prims == 0 means empty
     uint32_t lights_on;  // bit i: light i is switched on (Light::bLight)

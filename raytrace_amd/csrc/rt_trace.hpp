@@ -5,7 +5,8 @@
 // lower object index (an object replaces the running hit only when strictly
 // closer), and each primitive keeps its own acceptance rule and self-skip
 // rule.  The BVH only culls; its boxes are conservative (FP64-expanded leaf
-// bounds, robust slab test), so the result equals the scan's.
+// bounds, child boxes quantised outward, robust slab test), so the result
+// equals the scan's.
 //
 // Self-skip of triangles.  The reference tests the clTri *copies* of a
 // model's octant bins and skips only the copy whose address is the ray's
@@ -70,13 +71,14 @@ struct ArrayStack {
     RTG_HD void on_node() {}
     RTG_HD void on_prim() {}
     RTG_HD bool all_lanes(bool v) const { return v; }  // one lane
-    RTG_HD BvhNode node(const SceneView &S, int i) const { return S.nodes[i]; }
+    RTG_HD Bvh4Node node(const SceneView &S, int i) const { return S.nodes[i]; }
 };
 
 constexpr int kWalkDone = (int)0x80000000;  // no node left: below every leaf code
 
-// Visit the primitives of every leaf the segment [0, tmax] reaches, nearer
-// child first.  visit(prim_index, tmax&) returns true to stop.  The stack is
+// Visit the primitives of every leaf the segment [0, tmax] reaches, nearest
+// child first (kOrdered; otherwise in child order, for the any-hit test).
+// visit(prim_index, tmax&) returns true to stop.  The stack is
 // emptied on entry: an any-hit walk that stops early leaves entries behind,
 // and the stack only holds one entry per level of the current path (a tree
 // of depth < kStackMax) when every walk starts from an empty stack.
@@ -89,27 +91,60 @@ constexpr int kWalkDone = (int)0x80000000;  // no node left: below every leaf co
 // (all_lanes(v) == v) visits in exactly the order of the plain walk.  Only
 // for walks whose result does not depend on the order leaves are tested in
 // (the any-hit test; the closest hit, whose winner is the least (t, key)).
-template <bool kSpec = false, class Stack, class Visit>
+template <bool kSpec = false, bool kOrdered = true, class Stack, class Visit>
 RTG_HD void bvh_walk(const SceneView &S, const Ray &r, float &tmax, Stack &st, Visit &&visit) {
     st.reset();
     if (S.n_prims == 0) return;
     const V3 inv = safe_inv(r.d);
     int node = S.root;
-    auto step = [&](int nd) -> int {  // one internal node: the next node, pushing the far child
+    // One 4-wide node: the next node to enter (the nearest child the segment
+    // reaches), the other children it reaches pushed farthest first; the
+    // popped entry when it reaches none.
+    auto step = [&](int nd) -> int {
         st.on_node();
-        const BvhNode n = st.node(S, nd);
-        const float4 a = n.xy0, b = n.xy1, z = n.z01;
-        const int4 l = n.link;
-        float t0, t1;
-        const bool h0 = slab(r.o, inv, a.x, a.y, a.z, a.w, z.x, z.y, tmax, t0);
-        const bool h1 = slab(r.o, inv, b.x, b.y, b.z, b.w, z.z, z.w, tmax, t1);
-        if (h0 && h1) {
-            const bool first0 = t0 <= t1;
-            st.push(first0 ? l.y : l.x);
-            return first0 ? l.x : l.y;
+        const Bvh4Node n = st.node(S, nd);
+        const int nc = node4_count(n);
+        const int lk[4] = {n.link.x, n.link.y, n.link.z, n.link.w};
+        float t[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        bool h[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            float lx, hx, ly, hy, lz, hz;
+            node4_box(n, k, lx, hx, ly, hy, lz, hz);
+            h[k] = k < nc && slab(r.o, inv, lx, hx, ly, hy, lz, hz, tmax, t[k]);
         }
-        if (h0) return l.x;
-        if (h1) return l.y;
+        int next = kWalkDone;
+        if constexpr (kOrdered) {
+            // Sort keys: the entry distance's bits (t >= 0, so they order as
+            // the distances do) above the child's link, or ~0 for a child the
+            // segment misses; five compare-exchanges sort them, nearest first.
+            // The links travel in the keys: picking them by index afterwards
+            // would send the node to scratch memory.  Only the order of the
+            // walk depends on the sort.
+            uint64_t s[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                s[k] = h[k] ? ((uint64_t)__builtin_bit_cast(uint32_t, t[k] > 0.0f ? t[k] : 0.0f) << 32 |
+                               (uint32_t)lk[k])
+                            : ~0ull;
+            auto cx = [](uint64_t &a, uint64_t &b) {
+                const uint64_t lo = a < b ? a : b, hi = a < b ? b : a;
+                a = lo, b = hi;
+            };
+            cx(s[0], s[1]), cx(s[2], s[3]), cx(s[0], s[2]), cx(s[1], s[3]), cx(s[1], s[2]);
+#pragma unroll
+            for (int r = 3; r >= 1; --r)
+                if (s[r] != ~0ull) st.push((int)(uint32_t)s[r]);
+            if (s[0] != ~0ull) next = (int)(uint32_t)s[0];
+        } else {
+#pragma unroll
+            for (int k = 3; k >= 0; --k)
+                if (h[k]) {
+                    if (next != kWalkDone) st.push(next);
+                    next = lk[k];
+                }
+        }
+        if (next != kWalkDone) return next;
         return st.empty() ? kWalkDone : st.pop();
     };
     if constexpr (!kSpec) {
@@ -498,7 +533,7 @@ RTG_HD bool trace_any(const SceneView &S, const TraceRay &q, float dis, Stack &s
     if (brute_objects(S, q, true, dis, dummy)) return true;
     bool blocked = false;
     float tmax = dis;
-    bvh_walk<true>(S, q.r, tmax, st, [&](int pi, float &) -> bool {
+    bvh_walk<true, false>(S, q.r, tmax, st, [&](int pi, float &) -> bool {
         const PrimRec R = load_prim(S, pi);
         const float4 A = R.a;
         const int tag = as_int(A.w);

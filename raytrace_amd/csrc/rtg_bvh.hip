@@ -162,6 +162,27 @@ __global__ void k_top_first(const int *left, const int *right, const int *first,
     if (blockIdx.x == 0 && threadIdx.x == 0) top_first(left, right, first, last, n_int, newidx);
 }
 
+// Widening into 4-wide nodes (rt_bvh.hpp): parents of the binary nodes,
+// their depth parity, then one 4-wide node per binary node at even depth.
+__global__ void k_parent2(const BvhNode *nodes2, int n2, int *parent2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n2) return;
+    const int4 l = nodes2[i].link;
+    if (l.x >= 0) parent2[l.x] = i;
+    if (l.y >= 0) parent2[l.y] = i;
+}
+
+__global__ void k_even_flags(const int *parent2, int n2, int *flags) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n2) flags[i] = node2_even(parent2, i) ? 1 : 0;
+}
+
+__global__ void k_widen(const BvhNode *nodes2, const int *flags, const int *idx4, int n2, Bvh4Node *nodes4) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n2 || !flags[i]) return;
+    nodes4[idx4[i]] = widen_node(nodes2, idx4, i);
+}
+
 __global__ void k_fill_i32(int *p, int n, int v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -176,6 +197,34 @@ int upload(DevBuf &b, const std::vector<T> &v, hipStream_t s) {
     const size_t n = std::max<size_t>(v.size(), 1) * sizeof(T);
     if (ensure(b, n) != RTG_OK) return RTG_E_NOMEM;
     if (!v.empty()) RTG_HIP(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    return RTG_OK;
+}
+
+// The 4-wide tree of the n2 binary nodes in nodes2 (root 0) into S.nodes;
+// returns its node count in *n4.
+int widen_tree(const DevBuf &nodes2, int n2, SceneRes &S, hipStream_t stream, int *n4) {
+    DevBuf parent2, flags, idx4, scan_tmp;
+    if (ensure(parent2, n2 * sizeof(int)) || ensure(flags, n2 * sizeof(int)) || ensure(idx4, n2 * sizeof(int)))
+        return fail(RTG_E_NOMEM, "device allocation failed (4-wide nodes)");
+    k_fill_i32<<<nblk(n2), kBlock, 0, stream>>>(parent2.as<int>(), n2, -1);
+    k_parent2<<<nblk(n2), kBlock, 0, stream>>>(nodes2.as<BvhNode>(), n2, parent2.as<int>());
+    k_even_flags<<<nblk(n2), kBlock, 0, stream>>>(parent2.as<int>(), n2, flags.as<int>());
+    size_t tmp_bytes = 0;
+    RTG_HIP(rocprim::exclusive_scan(nullptr, tmp_bytes, flags.as<int>(), idx4.as<int>(), 0, (size_t)n2,
+                                    rocprim::plus<int>(), stream));
+    if (ensure(scan_tmp, std::max<size_t>(tmp_bytes, 16))) return fail(RTG_E_NOMEM, "scan temp");
+    RTG_HIP(rocprim::exclusive_scan(scan_tmp.p, tmp_bytes, flags.as<int>(), idx4.as<int>(), 0, (size_t)n2,
+                                    rocprim::plus<int>(), stream));
+    int last_idx = 0, last_flag = 0;
+    RTG_HIP(hipMemcpyAsync(&last_idx, idx4.as<int>() + (n2 - 1), sizeof(int), hipMemcpyDeviceToHost, stream));
+    RTG_HIP(hipMemcpyAsync(&last_flag, flags.as<int>() + (n2 - 1), sizeof(int), hipMemcpyDeviceToHost, stream));
+    RTG_HIP(hipStreamSynchronize(stream));
+    *n4 = last_idx + last_flag;
+    if (ensure(S.nodes, std::max(*n4, 1) * sizeof(Bvh4Node))) return fail(RTG_E_NOMEM, "bvh nodes");
+    k_widen<<<nblk(n2), kBlock, 0, stream>>>(nodes2.as<BvhNode>(), flags.as<int>(), idx4.as<int>(), n2,
+                                             S.nodes.as<Bvh4Node>());
+    RTG_HIP(hipGetLastError());
+    RTG_HIP(hipStreamSynchronize(stream));  // the temporaries are freed on return
     return RTG_OK;
 }
 
@@ -246,7 +295,7 @@ int scene_build(SceneRes &S, const rtg_scene_desc &d, hipStream_t stream, SceneR
         S.topo_n = donor->topo_n, S.topo_emit = donor->topo_emit;
         donor->topo_n = 0, donor->topo_emit = 0;
     }
-    DevBuf raw, mid, sph, prims_u, boxes, keys, flags, ibox, partial, cb, sort_tmp;
+    DevBuf raw, mid, sph, prims_u, boxes, keys, flags, ibox, partial, cb, sort_tmp, nodes2;
     DevBuf &keys2 = S.t_keys, &left = S.t_left, &right = S.t_right, &first = S.t_first, &last = S.t_last,
            &pint = S.t_pint, &pleaf = S.t_pleaf, &newidx = S.t_newidx;
     if ((st = upload(mid, part_mid, stream)) || (st = upload(sph, spheres, stream))) return st;
@@ -293,10 +342,10 @@ int scene_build(SceneRes &S, const rtg_scene_desc &d, hipStream_t stream, SceneR
         k_refit<<<nblk(n), kBlock, 0, stream>>>(sk, n, boxes.as<Box6>(), left.as<int>(), right.as<int>(), pint.as<int>(),
                                                 pleaf.as<int>(), flags.as<int>(), ibox.as<Box6>());
         n_emit = S.topo_emit;
-        if (ensure(S.nodes, std::max(n_emit, 1) * sizeof(BvhNode))) return fail(RTG_E_NOMEM, "bvh nodes");
+        if (ensure(nodes2, std::max(n_emit, 1) * sizeof(BvhNode))) return fail(RTG_E_NOMEM, "bvh nodes");
         k_emit<<<nblk(n - 1), kBlock, 0, stream>>>(sk, n - 1, boxes.as<Box6>(), left.as<int>(), right.as<int>(),
                                                    first.as<int>(), last.as<int>(), ibox.as<Box6>(), newidx.as<int>(),
-                                                   S.nodes.as<BvhNode>());
+                                                   nodes2.as<BvhNode>());
         if (ensure(S.prims, (size_t)n * 3 * sizeof(float4))) return fail(RTG_E_NOMEM, "prims");
         k_reorder<<<nblk(n), kBlock, 0, stream>>>(sk, n, prims_u.as<float4>(), S.prims.as<float4>());
         root = 0;
@@ -340,23 +389,25 @@ int scene_build(SceneRes &S, const rtg_scene_desc &d, hipStream_t stream, SceneR
         n_emit = last_new + last_flag;
         k_top_first<<<1, 1, 0, stream>>>(left.as<int>(), right.as<int>(), first.as<int>(), last.as<int>(), n - 1,
                                          newidx.as<int>());
-        if (ensure(S.nodes, std::max(n_emit, 1) * sizeof(BvhNode))) return fail(RTG_E_NOMEM, "bvh nodes");
+        if (ensure(nodes2, std::max(n_emit, 1) * sizeof(BvhNode))) return fail(RTG_E_NOMEM, "bvh nodes");
         k_emit<<<nblk(n - 1), kBlock, 0, stream>>>(sk, n - 1, boxes.as<Box6>(), left.as<int>(), right.as<int>(),
                                                    first.as<int>(), last.as<int>(), ibox.as<Box6>(), newidx.as<int>(),
-                                                   S.nodes.as<BvhNode>());
+                                                   nodes2.as<BvhNode>());
         if (ensure(S.prims, (size_t)n * 3 * sizeof(float4))) return fail(RTG_E_NOMEM, "prims");
         k_reorder<<<nblk(n), kBlock, 0, stream>>>(sk, n, prims_u.as<float4>(), S.prims.as<float4>());
         root = 0;
         S.topo_n = n, S.topo_emit = n_emit;
     } else {
         // few primitives: one leaf, prims in input order
-        if (ensure(S.prims, std::max(n, 1) * 3 * sizeof(float4)) || ensure(S.nodes, sizeof(BvhNode)))
+        if (ensure(S.prims, std::max(n, 1) * 3 * sizeof(float4)) || ensure(S.nodes, sizeof(Bvh4Node)))
             return fail(RTG_E_NOMEM, "prims");
         if (n > 0)
             RTG_HIP(hipMemcpyAsync(S.prims.p, prims_u.p, (size_t)n * 3 * sizeof(float4), hipMemcpyDeviceToDevice,
                                    stream));
         root = n > 0 ? make_leaf(0, n) : 0;
     }
+    int n4 = 0;
+    if (n_emit > 0 && (st = widen_tree(nodes2, n_emit, S, stream, &n4)) != RTG_OK) return st;
     RTG_HIP(hipEventRecord(e1, stream));
     RTG_HIP(hipStreamSynchronize(stream));
     float ms = 0;
@@ -364,13 +415,13 @@ int scene_build(SceneRes &S, const rtg_scene_desc &d, hipStream_t stream, SceneR
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     S.build_ms = ms;
-    S.n_nodes = n_emit;
+    S.n_nodes = n4;
     shading_state(S, d);
 
     SceneView &v = S.view;
     v = SceneView{};
-    v.nodes = S.nodes.as<BvhNode>();
-    v.n_nodes = n_emit;
+    v.nodes = S.nodes.as<Bvh4Node>();
+    v.n_nodes = n4;
     v.prims = S.prims.as<float4>();
     v.root = root;
     v.lights_on = S.lights_on;

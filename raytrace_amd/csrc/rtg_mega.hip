@@ -49,7 +49,7 @@ __device__ inline TraceRay primary_ray(const SceneView &S, const float *ax, cons
 
 __global__ __launch_bounds__(kBlockThreads) void k_megakernel(SceneView S, MegaArgs A) {
     __shared__ int lds[kLdsStack * kBlockThreads];
-    __shared__ BvhNode top_nodes[kTopNodes];
+    __shared__ Bvh4Node top_nodes[kTopNodes];
     const int tid = threadIdx.x;
     DevStack st = make_stack(lds, A.spill);
     stack_top_nodes(st, top_nodes, S);

@@ -5,7 +5,7 @@
 // HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
 // It renders with the megakernel, whose scratch is bounded by the persistent
 // grid rather than by the frame: the ray-generation tables (W + H floats) and
-// the traversal stacks' spill area (48 ints per thread of the grid).  The
+// the traversal stacks' spill area (80 ints per thread of the grid).  The
 // wavefront engine's queues grow with the frame (about 1 KB per sample at
 // depth 4, 2.2 GB for a 1080p frame), too much for a fixed array.
 //
@@ -27,7 +27,7 @@ constexpr size_t kCountsBytes = 256;
 constexpr size_t kTablesEnd = kCountsBytes + size_t(2) * RTG_MAX_DIM * sizeof(float);
 constexpr size_t kSpillPerBlock = size_t(kBlockThreads) * (kStackMax - kLdsStack) * sizeof(int);
 constexpr int kMaxBlocks = 2048;  // 8 per CU on a 256-CU MI355X
-constexpr size_t kScratchBytes = kTablesEnd + kMaxBlocks * kSpillPerBlock;  // 48 MiB + 128.25 KiB
+constexpr size_t kScratchBytes = kTablesEnd + kMaxBlocks * kSpillPerBlock;  // 80 MiB + 128.25 KiB
 
 constexpr int kMaxDevices = 64;
 std::mutex g_lock;

@@ -232,7 +232,7 @@ __global__ __launch_bounds__(kBlockThreads, 6) void k_extend(SceneView S0, WaveA
     __shared__ int lds[kLdsStack * kBlockThreads];
     DevStack st = make_stack(lds, A.spill);
     if constexpr (!B) {  // one scene: its top nodes in LDS
-        __shared__ BvhNode top_nodes[kTopNodes];
+        __shared__ Bvh4Node top_nodes[kTopNodes];
         stack_top_nodes(st, top_nodes, S0);
     }
     if (wave_cancelled(A)) return;
@@ -447,7 +447,7 @@ __global__ __launch_bounds__(kBlockThreads, 8) void k_occlude(SceneView S0, Wave
     __shared__ int lds[kLdsStack * kBlockThreads];
     DevStack st = make_stack(lds, A.spill);
     if constexpr (!B) {  // one scene: its top nodes in LDS
-        __shared__ BvhNode top_nodes[kTopNodes];
+        __shared__ Bvh4Node top_nodes[kTopNodes];
         stack_top_nodes(st, top_nodes, S0);
     }
     if (wave_cancelled(A)) return;

@@ -43,18 +43,20 @@ struct TrackStack {
     int pop() { return d[--sp]; }
     bool empty() const { return sp == 0; }
     bool all_lanes(bool v) const { return v; }
-    BvhNode node(const SceneView &S, int i) const { return S.nodes[i]; }
+    Bvh4Node node(const SceneView &S, int i) const { return S.nodes[i]; }
 };
 
 struct HostScene {
     ScenePrep prep;
-    std::vector<BvhNode> nodes;
+    std::vector<BvhNode> nodes2;  // the binary tree the 4-wide one is made from
+    std::vector<Bvh4Node> nodes;
     std::vector<float4> prims, tri_nrm;
     std::vector<float2> tri_uv;
     std::vector<float4> tri_pos;
     std::vector<uint8_t> tri_bins;
     SceneView view{};
-    int depth = 0;
+    int depth = 0;       // binary tree depth (leaf level included)
+    int depth4 = 0;      // 4-wide tree depth (leaf level included)
     int n_top = 0;       // nodes laid out first (top BFS levels)
     bool top_ok = true;  // emitted indices form a permutation with the root at 0
 };
@@ -137,12 +139,39 @@ int build(const rtg_scene_desc &d, HostScene &H) {
             }
             H.top_ok = ok && newidx[0] == 0;
         }
-        H.nodes.resize(std::max(n_emit, 1));
+        H.nodes2.resize(std::max(n_emit, 1));
         for (int i = 0; i < n - 1; ++i) {
             if (last[i] - first[i] + 1 <= kLeafMax) continue;
             auto bx = [&](int c) { return c < 0 ? boxes[(uint32_t)keys[~c]] : ibox[c]; };
-            H.nodes[newidx[i]] = make_node(bx(left[i]), bx(right[i]), emit_link(left[i], first.data(), last.data(), newidx.data()),
-                                           emit_link(right[i], first.data(), last.data(), newidx.data()));
+            H.nodes2[newidx[i]] = make_node(bx(left[i]), bx(right[i]), emit_link(left[i], first.data(), last.data(), newidx.data()),
+                                            emit_link(right[i], first.data(), last.data(), newidx.data()));
+        }
+        // widen, as the device does: parents, depth parity, prefix sum, nodes
+        std::vector<int> parent2(n_emit, -1), flags(n_emit), idx4(n_emit);
+        for (int i = 0; i < n_emit; ++i) {
+            const int4 l = H.nodes2[i].link;
+            if (l.x >= 0) parent2[l.x] = i;
+            if (l.y >= 0) parent2[l.y] = i;
+        }
+        int n4 = 0;
+        for (int i = 0; i < n_emit; ++i) {
+            flags[i] = node2_even(parent2.data(), i) ? 1 : 0;
+            idx4[i] = n4;
+            n4 += flags[i];
+        }
+        H.nodes.resize(std::max(n4, 1));
+        for (int i = 0; i < n_emit; ++i)
+            if (flags[i]) H.nodes[idx4[i]] = widen_node(H.nodes2.data(), idx4.data(), i);
+        // 4-wide depth
+        std::vector<std::pair<int, int>> q{{0, 1}};
+        while (!q.empty()) {
+            auto [i, dd] = q.back();
+            q.pop_back();
+            H.depth4 = std::max(H.depth4, dd + 1);
+            const Bvh4Node &nd = H.nodes[i];
+            const int lk[4] = {nd.link.x, nd.link.y, nd.link.z, nd.link.w};
+            for (int k = 0; k < node4_count(nd); ++k)
+                if (lk[k] >= 0) q.push_back({lk[k], dd + 1});
         }
         H.prims.resize((size_t)n * 3);
         for (int k = 0; k < n; ++k)
@@ -154,7 +183,7 @@ int build(const rtg_scene_desc &d, HostScene &H) {
     }
     SceneView &v = H.view;
     v.nodes = H.nodes.data();
-    v.n_nodes = (int)H.nodes.size();
+    v.n_nodes = n > kLeafMax ? (int)H.nodes.size() : 0;
     v.prims = H.prims.data();
     v.root = root;
     v.n_prims = n;
@@ -266,27 +295,45 @@ int emul_render(const rtg_scene_desc *d, const rtg_params *p, int threads, const
     return 0;
 }
 
-// BVH layout check of a scene: out = {emitted nodes, top nodes laid out
-// first, 1 if the emitted indices form a permutation with the root at 0,
-// 1 if the top nodes are the first BFS levels: every top node's emitted
-// children are top nodes or the top is full}.
+// BVH layout check of a scene: out = {4-wide nodes, binary nodes laid out
+// first (top BFS levels), 1 if the binary layout is a permutation with the
+// root at 0, 1 if the first min(kTopNodes, nodes) 4-wide nodes are the top of
+// the 4-wide tree (every one's parent among them) and every 4-wide node is
+// the child of exactly one other (root 0 of none)}.
 int emul_top_layout(const rtg_scene_desc *d, int32_t out[4]) {
     HostScene H;
     if (build(*d, H) != 0) return -1;
     out[0] = H.view.n_nodes;
     out[1] = H.n_top;
     out[2] = H.top_ok ? 1 : 0;
-    // BFS property on the emitted tree: nodes 0 .. n_top - 1 are closed under
-    // "parent of" (a top node's parent is a top node), and the root is 0
     bool ok = true;
-    std::vector<int> parent(H.nodes.size(), -1);
-    for (size_t i = 0; i < H.nodes.size(); ++i) {
-        const int4 l = H.nodes[i].link;
-        if (l.x >= 0) parent[l.x] = (int)i;
-        if (l.y >= 0) parent[l.y] = (int)i;
+    const int n4 = H.view.n_nodes;
+    std::vector<int> parent(std::max(n4, 1), -1), refs(std::max(n4, 1), 0);
+    for (int i = 0; i < n4; ++i) {
+        const Bvh4Node &nd = H.nodes[i];
+        const int lk[4] = {nd.link.x, nd.link.y, nd.link.z, nd.link.w};
+        for (int k = 0; k < node4_count(nd); ++k)
+            if (lk[k] >= 0) {
+                ok = ok && lk[k] < n4;
+                if (lk[k] < n4) parent[lk[k]] = i, ++refs[lk[k]];
+            }
     }
-    for (int i = 1; i < H.n_top && i < (int)H.nodes.size(); ++i) ok = ok && parent[i] >= 0 && parent[i] < H.n_top;
+    for (int i = 0; i < n4; ++i) ok = ok && refs[i] == (i == 0 ? 0 : 1);
+    const int top = std::min(kTopNodes, n4);
+    for (int i = 1; i < top; ++i) ok = ok && parent[i] >= 0 && parent[i] < top;
     out[3] = ok ? 1 : 0;
+    return 0;
+}
+
+// Shape of the scene's trees: out = {radix-tree depth, 4-wide depth, emitted
+// binary nodes, 4-wide nodes}.
+int emul_bvh_depths(const rtg_scene_desc *d, int32_t out[4]) {
+    HostScene H;
+    if (build(*d, H) != 0) return -1;
+    out[0] = H.depth;
+    out[1] = H.depth4;
+    out[2] = (int32_t)H.nodes2.size();
+    out[3] = (int32_t)H.nodes.size();
     return 0;
 }
 

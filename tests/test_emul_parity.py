@@ -72,15 +72,30 @@ def test_refraction_spp4_terrain_scene():
 def test_traversal_stack_stays_within_device_bound():
     """Regression: an any-hit walk that stops early used to leave entries on the
     per-thread stack, so later walks started deeper and the stack reached 64
-    entries (the device stack's capacity) on this scene.  Every walk now starts
-    empty and the stack never exceeds the tree depth."""
+    entries (the device stack's capacity then) on this scene.  Every walk now
+    starts empty, and a walk of the 4-wide tree leaves at most three entries
+    per level of its path."""
+    import ctypes as C
+    from conftest import emul_lib
     hs = rt.HostScene.workload(5, width=480, height=270, detail=8, n_spheres=64, n_lights=16)
     p = rt.params(480, 270, max_level=8, spp=4, seed=5)
     tiles = sample_tiles(480, 270, 4)
     e = emul_render(hs.desc_ptr, p, tiles=tiles)
-    assert e["depth"] <= 63  # Karras tree over 62-bit unique keys
-    assert e["max_stack"] < e["depth"]
-    assert e["max_stack"] <= 32  # fits the LDS part of the device stack on this scene
+    L = emul_lib()
+    L.emul_bvh_depths.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
+    d = (C.c_int32 * 4)()
+    assert L.emul_bvh_depths(C.c_void_p(hs.desc_ptr), d) == 0
+    depth2, depth4, n2, n4 = d[0], d[1], d[2], d[3]
+    assert n4 < n2  # one 4-wide node per binary node at even depth
+    assert depth2 == e["depth"]
+    assert depth2 <= 63  # Karras tree over 62-bit unique keys
+    # a 4-wide node spans two levels of the emitted binary tree, whose
+    # internal levels are at most those of the radix tree (depth - 1)
+    half_up = lambda levels: (levels + 1) // 2
+    assert depth4 - 1 <= half_up(depth2 - 1)
+    assert e["max_stack"] <= 3 * (depth4 - 1)
+    assert 3 * half_up(63 - 1) <= 96  # the device stack's capacity (kStackMax) holds any such tree
+    assert e["max_stack"] <= 16  # fits the LDS part of the device stack on this scene
 
 
 def test_tile_compat_leaves_partial_tiles():
