@@ -129,6 +129,20 @@ RTG_HD uint32_t expand_bits10(uint32_t v) {
     return v;
 }
 
+// Morton grid of the centroid bounds cb (lo x, y, z, hi x, y, z): origin and
+// scale.  The cells are cubes, the longest axis split into 1024: a grid
+// stretched to the bounds on every axis would split a flat scene (a height
+// field) across its thin axis as often as along the others, and its tree
+// visits 25% more nodes (CPU count, height field of workload 3).
+RTG_HD void morton_grid(const float cb[6], float cmin[3], float cinv[3]) {
+    float m = 0.0f;
+    for (int k = 0; k < 3; ++k) m = fmaxf(m, cb[3 + k] - cb[k]);
+    for (int k = 0; k < 3; ++k) {
+        cmin[k] = cb[k];
+        cinv[k] = m > 0 ? 1.0f / m : 0.0f;
+    }
+}
+
 RTG_HD uint64_t morton_key(const Box6 &b, const float cmin[3], const float cinv[3], int id) {
     uint32_t q[3];
     for (int k = 0; k < 3; ++k) {

@@ -72,11 +72,7 @@ __global__ void k_morton(const Box6 *boxes, int n, const float *cb, uint64_t *ke
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float cmin[3], cinv[3];
-    for (int k = 0; k < 3; ++k) {
-        cmin[k] = cb[k];
-        const float e = cb[3 + k] - cb[k];
-        cinv[k] = e > 0 ? 1.0f / e : 0.0f;
-    }
+    morton_grid(cb, cmin, cinv);
     keys[i] = morton_key(boxes[i], cmin, cinv, i);
 }
 

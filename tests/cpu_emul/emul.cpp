@@ -94,11 +94,7 @@ int build(const rtg_scene_desc &d, HostScene &H) {
                 cb[3 + k] = std::max(cb[3 + k], c);
             }
         float cmin[3], cinv[3];
-        for (int k = 0; k < 3; ++k) {
-            cmin[k] = cb[k];
-            const float e = cb[3 + k] - cb[k];
-            cinv[k] = e > 0 ? 1.0f / e : 0.0f;
-        }
+        morton_grid(cb, cmin, cinv);
         std::vector<uint64_t> keys(n);
         for (int i = 0; i < n; ++i) keys[i] = morton_key(boxes[i], cmin, cinv, i);
         std::sort(keys.begin(), keys.end());
