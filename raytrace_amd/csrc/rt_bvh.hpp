@@ -24,7 +24,7 @@
 
 namespace rtg {
 
-constexpr int kLeafMax = 4;
+constexpr int kLeafMax = 2;  // primitives per leaf at most
 
 struct Box6 {
     float lo[3], hi[3];
