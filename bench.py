@@ -83,6 +83,48 @@ def pmc_traffic(engine: str, kernel: str):
         return None
 
 
+def pmc_summary(d: str, engine: str = "wave") -> dict:
+    """HBM bytes per dispatch by kernel from the rocprofv3 --pmc CSVs under d:
+    copy_fetch/ and copy_write/ (FETCH_SIZE, WRITE_SIZE of a --calibrate-copy
+    run) and <engine>_fetch/, <engine>_write/ (the same counters over frames).
+    Each counter is scaled by what it reads for the plain copy: 2^30 bytes read
+    and written per copy dispatch."""
+    import csv
+    import glob
+    import re
+    from collections import defaultdict
+
+    def per_kernel(sub, counter):
+        total, disp = defaultdict(float), defaultdict(set)
+        for path in glob.glob(os.path.join(d, sub, "**", "*counter_collection.csv"), recursive=True):
+            with open(path) as f:
+                for r in csv.DictReader(f):
+                    if r["Counter_Name"] != counter:
+                        continue
+                    m = re.search(r"(k_\w+|__amd_rocclr_\w+)", r["Kernel_Name"])
+                    k = m.group(1) if m else r["Kernel_Name"]
+                    total[k] += float(r["Counter_Value"])
+                    disp[k].add(r["Dispatch_Id"])
+        return {k: (total[k] / len(disp[k]), len(disp[k])) for k in total}
+
+    copy_kib = (1 << 30) / 1024.0
+    cf = per_kernel("copy_fetch", "FETCH_SIZE")
+    cw = per_kernel("copy_write", "WRITE_SIZE")
+    copy_k = [k for k in cf if "copy" in k.lower()]
+    if not copy_k:
+        raise RuntimeError(f"no copy kernel under {d}/copy_fetch")
+    fetch_factor = copy_kib / cf[copy_k[0]][0]
+    write_factor = copy_kib / cw[copy_k[0]][0]
+    fetch, write = per_kernel(f"{engine}_fetch", "FETCH_SIZE"), per_kernel(f"{engine}_write", "WRITE_SIZE")
+    out = {}
+    for k in sorted(set(fetch) & set(write)):
+        out[k] = {"hbm_bytes_per_launch": int(round((fetch[k][0] * fetch_factor + write[k][0] * write_factor) * 1024)),
+                  "fetch_kib_raw": round(fetch[k][0], 1), "write_kib_raw": round(write[k][0], 1),
+                  "dispatches": fetch[k][1]}
+    return {engine: out, "calibration": {"fetch_factor": round(fetch_factor, 4), "write_factor": round(write_factor, 4),
+                                         "copy_bytes": 1 << 30}}
+
+
 def batch_scenes(rt):
     """The 8 resident scenes of the batch and serving workloads (triangle soups
     and height fields of 1k..1M triangles, 1-8 lights, depth 2-6)."""
@@ -354,7 +396,13 @@ def main() -> int:
                     help="workload 2: every kernel on the render stream (RTG_FLAG_ONE_STREAM), for counter collection")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (reference for FETCH_SIZE / WRITE_SIZE scaling)")
+    ap.add_argument("--pmc-summary", metavar="DIR",
+                    help="only print HBM bytes per dispatch by kernel from the rocprofv3 --pmc runs under DIR "
+                         "(copy_fetch, copy_write, wave_fetch, wave_write)")
     args = ap.parse_args()
+    if args.pmc_summary:
+        print(json.dumps(pmc_summary(args.pmc_summary), indent=1))
+        return 0
     if args.calibrate_copy:
         import torch
         a = torch.empty(1 << 30, dtype=torch.uint8, device="cuda:0")

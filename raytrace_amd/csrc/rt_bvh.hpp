@@ -274,6 +274,11 @@ RTG_HD int node4_children(const BvhNode *nodes2, int i, Box6 box[4], int link[4]
     return c;
 }
 
+RTG_HD double box_area(const Box6 &b) {
+    const double x = (double)b.hi[0] - b.lo[0], y = (double)b.hi[1] - b.lo[1], z = (double)b.hi[2] - b.lo[2];
+    return x * y + y * z + z * x;
+}
+
 // Depth parity of binary node i (root 0 at depth 0) by its parent links.
 RTG_HD bool node2_even(const int *parent2, int i) {
     int d = 0;
@@ -338,21 +343,15 @@ RTG_HD Bvh4Node make_node4(const Box6 *box, const int *link, int nc) {
     return n;
 }
 
-// The 4-wide node made from binary node i (at even depth); idx4 maps binary
-// node indices at even depth to 4-wide indices.  The any-hit walk takes the
-// children in stored order (closest-hit walks sort them by distance), so the
-// order is chosen to meet an occluder early: leaves first, the smaller leaves
-// first, then internal children by surface area, the largest first.
-RTG_HD Bvh4Node widen_node(const BvhNode *nodes2, const int *idx4, int i) {
-    Box6 box[4];
-    int link[4];
-    const int nc = node4_children(nodes2, i, box, link);
+// The 4-wide node over `nc` children with final links.  The any-hit walk
+// takes the children in stored order (closest-hit walks sort them by
+// distance), so the order is chosen to meet an occluder early: leaves first,
+// the smaller leaves first, then internal children by surface area, the
+// largest first.
+RTG_HD Bvh4Node ordered_node4(Box6 box[4], int link[4], int nc) {
     double key[4];
-    for (int c = 0; c < nc; ++c) {
-        const Box6 &b = box[c];
-        const double x = (double)b.hi[0] - b.lo[0], y = (double)b.hi[1] - b.lo[1], z = (double)b.hi[2] - b.lo[2];
-        key[c] = link[c] < 0 ? leaf_count(link[c]) - 8.0 : 1.0 / (1.0 + (x * y + y * z + z * x));
-    }
+    for (int c = 0; c < nc; ++c)
+        key[c] = link[c] < 0 ? leaf_count(link[c]) - 8.0 : 1.0 / (1.0 + box_area(box[c]));
     for (int a = 1; a < nc; ++a)  // stable insertion sort, ascending key
         for (int b = a; b > 0 && key[b] < key[b - 1]; --b) {
             const double tk = key[b];
@@ -362,9 +361,18 @@ RTG_HD Bvh4Node widen_node(const BvhNode *nodes2, const int *idx4, int i) {
             const int tl = link[b];
             link[b] = link[b - 1], link[b - 1] = tl;
         }
+    return make_node4(box, link, nc);
+}
+
+// The 4-wide node made from binary node i (at even depth); idx4 maps binary
+// node indices at even depth to 4-wide indices.
+RTG_HD Bvh4Node widen_node(const BvhNode *nodes2, const int *idx4, int i) {
+    Box6 box[4];
+    int link[4];
+    const int nc = node4_children(nodes2, i, box, link);
     for (int c = 0; c < nc; ++c)
         if (link[c] >= 0) link[c] = idx4[link[c]];
-    return make_node4(box, link, nc);
+    return ordered_node4(box, link, nc);
 }
 
 RTG_HD BvhNode make_node(const Box6 &b0, const Box6 &b1, int l0, int l1) {

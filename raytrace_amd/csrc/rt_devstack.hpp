@@ -12,6 +12,7 @@ constexpr int kLdsStack = 16;       // entries per thread in LDS (8 KB per block
 // A Karras tree over 62-bit unique keys is fewer than 64 binary levels deep,
 // so its 4-wide form has at most 32 levels, each leaving at most 3 entries.
 constexpr int kStackMax = 96;
+static_assert(3 * kMaxLevels4 <= kStackMax, "the stack holds any walk of a tree the build emits");
 
 // Per-thread stack: the first kLdsStack entries in LDS (thread-interleaved, no
 // bank conflicts), the rest in a per-thread global spill area.

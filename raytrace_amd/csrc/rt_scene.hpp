@@ -42,6 +42,11 @@ struct alignas(16) Bvh4Node {
 };
 static_assert(sizeof(Bvh4Node) == 64, "64-byte node");
 
+// Internal levels of a 4-wide tree the traversal stack holds: a walk leaves at
+// most 3 entries per level.  A Karras tree over 62-bit keys has under 63
+// internal binary levels, so its even-depth widening stays within this.
+constexpr int kMaxLevels4 = 32;
+
 // The first nodes are the top levels of the tree (indices < kTopNodes):
 // traversal kernels that render one scene keep them in LDS.
 constexpr int kTopNodes = 31;
