@@ -357,4 +357,27 @@ int emul_batch_group_end(const unsigned long long *samples, int n, int begin, un
     return rtg::batch_group_end(s.data(), n, begin, (size_t)cap);
 }
 
+// One 4-wide node over `nc` (2..4) children (boxes lo x, y, z, hi x, y, z
+// per child; links), as the build makes it (ordered_node4), read back as the
+// traversal decodes it: per stored child its box and link.  Returns the
+// stored child count.
+int emul_node4(const float *boxes, const int32_t *links, int nc, float *decoded, int32_t *out_links) {
+    if (nc < 2 || nc > 4) return -1;
+    Box6 b[4];
+    int l[4];
+    for (int c = 0; c < nc; ++c) {
+        for (int a = 0; a < 3; ++a) b[c].lo[a] = boxes[6 * c + a], b[c].hi[a] = boxes[6 * c + 3 + a];
+        l[c] = links[c];
+    }
+    const Bvh4Node n = ordered_node4(b, l, nc);
+    const int lk[4] = {n.link.x, n.link.y, n.link.z, n.link.w};
+    const int m = node4_count(n);
+    for (int k = 0; k < m; ++k) {
+        float *d = decoded + 6 * k;
+        node4_box(n, k, d[0], d[3], d[1], d[4], d[2], d[5]);
+        out_links[k] = lk[k];
+    }
+    return m;
+}
+
 }  // extern "C"

@@ -251,3 +251,57 @@ def test_refracting_sphere_children_identity():
         _assert_parity(o, e)
         assert list(e["counts"]) == [o["stats"].rays_primary, o["stats"].rays_shadow, o["stats"].rays_reflect,
                                      o["stats"].rays_refract]
+
+
+def test_quantised_child_boxes_contain_the_exact_ones():
+    """A 4-wide node stores its children's boxes as bytes on a per-axis
+    power-of-two grid, rounded outward (rt_bvh.hpp quantize_axis): every
+    decoded box must contain its child's exact box, or the walk could cull a
+    primitive the reference hits.  Random boxes over 60 decades of magnitude,
+    zero and one-ulp extents, identical children, huge offsets from the
+    node's corner, and infinite bounds (the full-range fallback, which
+    decodes an infinite lower bound to -FLT_MAX: no finite point lies below
+    it, so the comparison clips the exact box to the finite range)."""
+    import ctypes as C
+    from conftest import emul_lib
+    L = emul_lib()
+    L.emul_node4.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+    rng = np.random.default_rng(21)
+    dec = np.zeros(24, np.float32)
+    out_links = np.zeros(4, np.int32)
+    checked = 0
+    for trial in range(20000):
+        nc = int(rng.integers(2, 5))
+        kind = trial % 6
+        scale = np.float32(10.0 ** rng.uniform(-30, 30))
+        centre = (rng.standard_normal(3) * scale * rng.choice([0.0, 1.0, 1e3])).astype(np.float32)
+        lo = (centre + rng.standard_normal((nc, 3)).astype(np.float32) * scale).astype(np.float32)
+        ext = np.abs(rng.standard_normal((nc, 3))).astype(np.float32) * scale
+        if kind == 1:
+            ext[:] = 0  # points
+        elif kind == 2:
+            ext = np.abs(np.spacing(lo)).astype(np.float32)  # one ulp
+        elif kind == 3:
+            lo[:] = lo[0]  # identical children
+            ext[:] = ext[0]
+        hi = (lo + ext).astype(np.float32)
+        hi = np.maximum(hi, lo)
+        if kind == 4:  # one child far from the others
+            lo[-1] += np.float32(1e6) * scale
+            hi[-1] = np.maximum(hi[-1] + np.float32(1e6) * scale, lo[-1])
+        if kind == 5 and trial % 12 == 5:
+            lo[0, 0] = -np.inf
+            hi[0, 1] = np.inf
+        assert np.isfinite(lo[np.isfinite(lo)]).all()
+        boxes = np.ascontiguousarray(np.concatenate([lo, hi], axis=1).astype(np.float32))
+        links = np.arange(1, nc + 1, dtype=np.int32) * np.where(rng.random(nc) < 0.5, 1, -1).astype(np.int32)
+        m = L.emul_node4(boxes.ctypes.data, links.ctypes.data, nc, dec.ctypes.data, out_links.ctypes.data)
+        assert m == nc
+        assert sorted(out_links[:m].tolist()) == sorted(links.tolist())
+        for c in range(nc):
+            k = int(np.where(out_links[:m] == links[c])[0][0])
+            d = dec[6 * k:6 * k + 6]
+            e = np.clip(boxes[c], -np.finfo(np.float32).max, np.finfo(np.float32).max)
+            assert (d[:3] <= e[:3]).all() and (d[3:] >= e[3:]).all(), (trial, boxes[c], d)
+        checked += 1
+    assert checked == 20000
