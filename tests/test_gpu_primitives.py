@@ -133,3 +133,54 @@ def test_loaded_obj_mtl_bmp_asset(rt_gpu, engine, tmp_path):
     ot = _oracle(hs, pt)
     _check(gt, ot)
     assert s["rgb_max"] <= 1
+
+
+def _spheres_scene(rt, centres, radius=0.6, reflect=0.3):
+    hs = rt.HostScene.new()
+    for i, c in enumerate(centres):
+        s = hs.add_sphere(radius)
+        hs.set_position(s, tuple(float(x) for x in c))
+        hs.set_material(s, (0.1, 0.1, 0.1), (0.3 + 0.05 * (i % 8), 0.4, 0.6 - 0.05 * (i % 8)), (0.5, 0.5, 0.5), 24.0,
+                        reflect=reflect)
+    hs.add_light(2, (0.2, 0.5, 0.3), (1, 0, 0.01, 1))
+    hs.set_light_position(0, (3, 8, 6, 1))
+    return hs
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 9])
+def test_small_trees(rt_gpu, n):
+    """Scenes of 1..9 spheres: a lone leaf (n <= 2) and the smallest 4-wide
+    trees, nodes of 2, 3 and 4 children, leaves of one and two primitives."""
+    rt = rt_gpu
+    log(f"case: {n} spheres, 320x240 depth 3, both engines")
+    rng = np.random.default_rng(100 + n)
+    hs = _spheres_scene(rt, rng.uniform(-3, 3, (n, 3)))
+    p = rt.params(320, 240, max_level=3)
+    o = _oracle(hs, p)
+    ds = rt.DeviceScene(hs)
+    for engine in ENGINES:
+        p.engine = engine
+        s = _check(ds.render(p), o)
+        assert s["rgb_max"] == 0 and s["t_exact"] == 1.0
+    assert (o["ids"][..., 0] >= 0).sum() > 500
+
+
+def test_coincident_primitives(rt_gpu):
+    """Many spheres at one point share one Morton code (the radix tree splits
+    them by index) and tie on every hit: the lowest object index wins
+    (RayTracer.cpp:455-465).  Plus a row of identical spheres far from the
+    others, so one node's children span very different extents."""
+    rt = rt_gpu
+    log("case: 12 coincident spheres + a distant row, 320x240 depth 2, both engines")
+    centres = [(0.0, 1.0, 0.0)] * 12 + [(40.0 + 0.01 * k, -2.0, -30.0) for k in range(6)]
+    hs = _spheres_scene(rt, np.array(centres), radius=0.9, reflect=0.2)
+    p = rt.params(320, 240, max_level=2)
+    o = _oracle(hs, p)
+    ds = rt.DeviceScene(hs)
+    for engine in ENGINES:
+        p.engine = engine
+        g = ds.render(p)
+        s = _check(g, o)
+        assert s["rgb_max"] == 0
+    ids = o["ids"][..., 0]
+    assert (ids == 0).sum() > 500 and not np.isin(ids, np.arange(1, 12)).any()
