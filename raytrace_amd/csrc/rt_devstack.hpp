@@ -56,7 +56,11 @@ struct DevStack {
     __device__ bool empty() const { return sp == 0; }
     __device__ void on_node() {}
     __device__ void on_prim() {}
-    __device__ bool all_lanes(bool v) const { return __all(v); }  // over the wave's active lanes
+    // v holds on at least N/D of the wave's active lanes
+    template <int N, int D>
+    __device__ bool lanes_at_least(bool v) const {
+        return D * __popcll(__ballot(v)) >= N * __popcll(__ballot(true));
+    }
     __device__ Bvh4Node node(const SceneView &S, int i) const {
 #if defined(__HIP_DEVICE_COMPILE__)
         typedef const Bvh4Node __attribute__((address_space(3))) LdsNode;  // `top` points into LDS

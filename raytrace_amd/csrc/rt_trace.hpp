@@ -70,7 +70,10 @@ struct ArrayStack {
     RTG_HD bool empty() const { return sp == 0; }
     RTG_HD void on_node() {}
     RTG_HD void on_prim() {}
-    RTG_HD bool all_lanes(bool v) const { return v; }  // one lane
+    template <int N, int D>
+    RTG_HD bool lanes_at_least(bool v) const {  // one lane
+        return v;
+    }
     RTG_HD Bvh4Node node(const SceneView &S, int i) const { return S.nodes[i]; }
 };
 
@@ -85,12 +88,14 @@ constexpr int kWalkDone = (int)0x80000000;  // no node left: below every leaf co
 //
 // kSpec: speculative traversal (Aila and Laine 2009).  A lane that reaches a
 // leaf postpones it and keeps walking nodes, postponing a second leaf too,
-// until it reaches a third; the node loop ends when every lane of the wave
-// holds a leaf (Stack::all_lanes), then the held leaves are tested.  The lanes of a wave
-// then spend fewer steps waiting for one another.  A single lane
-// (all_lanes(v) == v) visits in exactly the order of the plain walk.  Only
-// for walks whose result does not depend on the order leaves are tested in
-// (the any-hit test; the closest hit, whose winner is the least (t, key)).
+// until it reaches a third; the node loop ends when enough lanes of the wave
+// hold a leaf (Stack::lanes_at_least: half of them in the closest-hit walk,
+// three quarters in the any-hit walk), then the held leaves are tested.  The
+// lanes of a wave then spend fewer steps waiting for one another.  A single
+// lane (lanes_at_least(v) == v) visits in exactly the order of the plain
+// walk.  Only for walks whose result does not depend on the order leaves are
+// tested in (the any-hit test; the closest hit, whose winner is the least
+// (t, key)).
 template <bool kSpec = false, bool kOrdered = true, class Stack, class Visit>
 RTG_HD void bvh_walk(const SceneView &S, const Ray &r, float &tmax, Stack &st, Visit &&visit) {
     st.reset();
@@ -170,7 +175,10 @@ RTG_HD void bvh_walk(const SceneView &S, const Ray &r, float &tmax, Stack &st, V
                     (leaf == 0 ? leaf : leaf2) = node;
                     node = st.empty() ? kWalkDone : st.pop();
                 }
-                if (st.all_lanes(leaf != 0)) break;
+                // closest hit: half the lanes; any hit: three quarters
+                if (kOrdered ? st.template lanes_at_least<1, 2>(leaf != 0)
+                             : st.template lanes_at_least<3, 4>(leaf != 0))
+                    break;
             }
             if (leaf == 0 && node < 0 && node != kWalkDone) {
                 leaf = node;

@@ -42,7 +42,10 @@ struct TrackStack {
     }
     int pop() { return d[--sp]; }
     bool empty() const { return sp == 0; }
-    bool all_lanes(bool v) const { return v; }
+    template <int N, int D>
+    bool lanes_at_least(bool v) const {
+        return v;
+    }
     Bvh4Node node(const SceneView &S, int i) const { return S.nodes[i]; }
 };
 
