@@ -296,9 +296,9 @@ int emul_render(const rtg_scene_desc *d, const rtg_params *p, int threads, const
 
 // BVH layout check of a scene: out = {4-wide nodes, binary nodes laid out
 // first (top BFS levels), 1 if the binary layout is a permutation with the
-// root at 0, 1 if the first min(kTopNodes, nodes) 4-wide nodes are the top of
-// the 4-wide tree (every one's parent among them) and every 4-wide node is
-// the child of exactly one other (root 0 of none)}.
+// root at 0, 1 if every 4-wide node is the child of exactly one other (root
+// 0 of none) and the first 4-wide nodes are exactly the nodes of the top
+// three 4-wide levels}.
 int emul_top_layout(const rtg_scene_desc *d, int32_t out[4]) {
     HostScene H;
     if (build(*d, H) != 0) return -1;
@@ -318,8 +318,17 @@ int emul_top_layout(const rtg_scene_desc *d, int32_t out[4]) {
             }
     }
     for (int i = 0; i < n4; ++i) ok = ok && refs[i] == (i == 0 ? 0 : 1);
-    const int top = std::min(kTopNodes, n4);
-    for (int i = 1; i < top; ++i) ok = ok && parent[i] >= 0 && parent[i] < top;
+    if (ok) {
+        std::vector<int> depth(std::max(n4, 1), 0);
+        for (int i = 1; i < n4; ++i) {
+            int d = 0;
+            for (int j = i; j > 0 && d <= n4; j = parent[j]) ++d;
+            depth[i] = d;
+        }
+        int in_top3 = 0;
+        for (int i = 0; i < n4; ++i) in_top3 += depth[i] < 3;
+        for (int i = 0; i < n4; ++i) ok = ok && ((depth[i] < 3) == (i < in_top3));
+    }
     out[3] = ok ? 1 : 0;
     return 0;
 }

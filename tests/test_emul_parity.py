@@ -305,3 +305,27 @@ def test_quantised_child_boxes_contain_the_exact_ones():
             assert (d[:3] <= e[:3]).all() and (d[3:] >= e[3:]).all(), (trial, boxes[c], d)
         checked += 1
     assert checked == 20000
+
+
+def test_top_levels_are_laid_out_first():
+    """The traversal kernels of one scene copy the first 31 4-wide nodes into
+    LDS and read every node with a smaller index from there (any nodes would
+    be correct; the top ones are the ones every walk reads).  The build lays
+    out the binary tree's top five BFS levels first (top_first) and the
+    widening keeps that order, so the first 4-wide nodes are exactly the top
+    three 4-wide levels (up to 21 nodes; the rest of the 31 follow in
+    emission order).  Also: the binary layout is a permutation with the root
+    at 0, and every 4-wide node has exactly one parent."""
+    import ctypes as C
+    from conftest import emul_lib
+    L = emul_lib()
+    L.emul_top_layout.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
+    scenes = [rt.HostScene.workload(1), rt.HostScene.workload(2), rt.HostScene.workload(3, index=0),
+              rt.HostScene.workload(3, index=1), rt.HostScene.workload(5, width=256, height=144, detail=16)]
+    for hs in scenes:
+        out = (C.c_int32 * 4)()
+        assert L.emul_top_layout(C.c_void_p(hs.desc_ptr), out) == 0
+        n4, n_top, perm_ok, top_ok = list(out)
+        assert perm_ok == 1 and top_ok == 1, list(out)
+        if n4 >= 31:
+            assert n_top == 31  # five full binary levels
