@@ -58,7 +58,9 @@ RTG_HD int make_leaf(int first, int count) { return ~((first << 3) | (count - 1)
 
 RTG_HD int node4_count(const Bvh4Node &n) { return (int)(__builtin_bit_cast(uint32_t, n.org.w) >> 24); }
 RTG_HD float pow2f(int e) { return __builtin_bit_cast(float, (uint32_t)e << 23); }  // 2^(e - 127), e in 1..254
-RTG_HD float dequant(float org, uint32_t q, float s) { return org + (float)q * s; }
+// org + q s.  q s is exact (8 bits times a power of two), so a fused
+// multiply-add gives the same value as the product and the sum, one rounding.
+RTG_HD float dequant(float org, uint32_t q, float s) { return fmaf((float)q, s, org); }
 // Child k's decoded box: lo/hi per axis.
 RTG_HD void node4_box(const Bvh4Node &n, int k, float &lx, float &hx, float &ly, float &hy, float &lz, float &hz) {
     const uint32_t e = __builtin_bit_cast(uint32_t, n.org.w);

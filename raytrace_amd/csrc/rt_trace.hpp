@@ -101,6 +101,9 @@ RTG_HD void bvh_walk(const SceneView &S, const Ray &r, float &tmax, Stack &st, V
     st.reset();
     if (S.n_prims == 0) return;
     const V3 inv = safe_inv(r.d);
+    // per axis, whether the direction is negative: then the entry distance
+    // of a box on that axis comes from its upper bound
+    const bool nx = inv.x < 0.0f, ny = inv.y < 0.0f, nz = inv.z < 0.0f;
     int node = S.root;
     // One 4-wide node: the next node to enter (the nearest child the segment
     // reaches), the other children it reaches pushed farthest first; the
@@ -112,11 +115,28 @@ RTG_HD void bvh_walk(const SceneView &S, const Ray &r, float &tmax, Stack &st, V
         const int lk[4] = {n.link.x, n.link.y, n.link.z, n.link.w};
         float t[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         bool h[4];
+        // The slab test of `slab` on each child's decoded box, with the entry
+        // and exit bound of each axis picked by the direction's sign instead
+        // of ordering the two distances: the same values, since the distances
+        // keep the bounds' order when inv > 0 and reverse it when inv < 0.
+        const uint32_t e = __builtin_bit_cast(uint32_t, n.org.w);
+        const float sx = pow2f(e & 0xFF), sy = pow2f((e >> 8) & 0xFF), sz = pow2f((e >> 16) & 0xFF);
+        const uint32_t enx = nx ? n.qxy.y : n.qxy.x, fx = nx ? n.qxy.x : n.qxy.y;
+        const uint32_t eny = ny ? n.qxy.w : n.qxy.z, fy = ny ? n.qxy.z : n.qxy.w;
+        const uint32_t enz = nz ? n.qz.y : n.qz.x, fz = nz ? n.qz.x : n.qz.y;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            float lx, hx, ly, hy, lz, hz;
-            node4_box(n, k, lx, hx, ly, hy, lz, hz);
-            h[k] = k < nc && slab(r.o, inv, lx, hx, ly, hy, lz, hz, tmax, t[k]);
+            const int sh = 8 * k;
+            const float ax = (dequant(n.org.x, (enx >> sh) & 0xFF, sx) - r.o.x) * inv.x;
+            const float bx = (dequant(n.org.x, (fx >> sh) & 0xFF, sx) - r.o.x) * inv.x;
+            const float ay = (dequant(n.org.y, (eny >> sh) & 0xFF, sy) - r.o.y) * inv.y;
+            const float by = (dequant(n.org.y, (fy >> sh) & 0xFF, sy) - r.o.y) * inv.y;
+            const float az = (dequant(n.org.z, (enz >> sh) & 0xFF, sz) - r.o.z) * inv.z;
+            const float bz = (dequant(n.org.z, (fz >> sh) & 0xFF, sz) - r.o.z) * inv.z;
+            const float near = fmaxf(fmaxf(ax, ay), fmaxf(az, 0.0f));
+            const float far = fminf(fminf(bx, by), bz) * kFarScale;
+            t[k] = near;
+            h[k] = k < nc && near <= fminf(far, tmax);
         }
         int next = kWalkDone;
         if constexpr (kOrdered) {
