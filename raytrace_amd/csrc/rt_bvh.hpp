@@ -288,10 +288,12 @@ RTG_HD bool node2_even(const int *parent2, int i) {
 
 // Byte bounds of `nc` child boxes on one axis: the smallest power-of-two step
 // s = 2^(e - 127) for which every child's [lo, hi] is covered by
-// [org + qlo s, org + qhi s] with org the union's lower bound and q in 0..255,
-// the covering checked with the decode's own float arithmetic.  An axis no
-// step covers (infinite bounds) gets org = -FLT_MAX, s = 2^127 and the full
-// byte range, which decodes to [-FLT_MAX, +inf].
+// [org + qlo s, org + qhi s] with q in 0..255 and org the union's lower bound
+// rounded down to a multiple m s of the step, |m| + 255 < 2^24: then every
+// decoded bound (m + q) s is a float exactly, which the walk's child test
+// relies on (Node4Test).  The covering is checked with the decode's own float
+// arithmetic.  An axis no step covers (infinite bounds) gets org = -FLT_MAX,
+// s = 2^127 and the full byte range, which decodes to [-FLT_MAX, +inf].
 RTG_HD void quantize_axis(const Box6 *b, int nc, int a, float &org, int &e, uint8_t *qlo, uint8_t *qhi) {
     float lo = b[0].lo[a], hi = b[0].hi[a];
     for (int c = 1; c < nc; ++c) lo = fminf(lo, b[c].lo[a]), hi = fmaxf(hi, b[c].hi[a]);
@@ -305,19 +307,22 @@ RTG_HD void quantize_axis(const Box6 *b, int nc, int a, float &org, int &e, uint
     }
     for (; k <= 127; ++k) {
         const float s = pow2f(k + 127);
+        const double mo = floor((double)lo / (double)s);  // exact: lo / s is a power-of-two scaling
+        if (!(fabs(mo) + 255.0 < 16777216.0)) continue;
+        const float o = (float)(mo * (double)s);  // m s, exact
         bool ok = true;
         for (int c = 0; c < nc && ok; ++c) {
-            const double fl = floor(((double)b[c].lo[a] - (double)lo) / (double)s);
-            const double fh = ceil(((double)b[c].hi[a] - (double)lo) / (double)s);
+            const double fl = floor(((double)b[c].lo[a] - (double)o) / (double)s);
+            const double fh = ceil(((double)b[c].hi[a] - (double)o) / (double)s);
             int ql = fl < 0 ? 0 : (fl > 255 ? 255 : (int)fl);
             int qh = fh < 0 ? 0 : (fh > 255 ? 255 : (int)fh);
-            while (ql > 0 && !(dequant(lo, (uint32_t)ql, s) <= b[c].lo[a])) --ql;
-            while (qh < 255 && !(dequant(lo, (uint32_t)qh, s) >= b[c].hi[a])) ++qh;
-            ok = dequant(lo, (uint32_t)ql, s) <= b[c].lo[a] && dequant(lo, (uint32_t)qh, s) >= b[c].hi[a];
+            while (ql > 0 && !(dequant(o, (uint32_t)ql, s) <= b[c].lo[a])) --ql;
+            while (qh < 255 && !(dequant(o, (uint32_t)qh, s) >= b[c].hi[a])) ++qh;
+            ok = dequant(o, (uint32_t)ql, s) <= b[c].lo[a] && dequant(o, (uint32_t)qh, s) >= b[c].hi[a];
             qlo[c] = (uint8_t)ql, qhi[c] = (uint8_t)qh;
         }
         if (ok) {
-            org = lo, e = k + 127;
+            org = o, e = k + 127;
             return;
         }
     }

@@ -60,6 +60,51 @@ RTG_HD bool slab(V3 o, V3 inv, float lx, float hx, float ly, float hy, float lz,
     return n <= fminf(f, tmax);
 }
 
+// The slab test of a ray against the children of one 4-wide node, at most as
+// strict as `slab` on the decoded boxes (it never culls a child `slab` keeps).
+// Each axis's entry and exit bound is picked by the direction's sign, and its
+// distance is one fused multiply-add, q (s inv) + (org - o) inv, with s inv
+// exact (s is a power of two).  The decoded bound org + q s = (m + q) s is a
+// float exactly (quantize_axis), so against `slab`'s fl(fl(org + q s - o) inv)
+// the result is off by at most 6.1 eps |(org - o) inv| + 766 eps |s inv|
+// (eps = 2^-24, the constant term's own rounding included); the constant term
+// moves out by 16 eps |(org - o) inv| + 2048 eps |s inv|, and rounding to
+// nearest keeps the margin's effect monotone.  An overflow makes a term
+// infinite or NaN, which fmaxf / fminf drop: the test then only passes more
+// children.
+struct Node4Test {
+    float sx, sy, sz;  // s inv per axis
+    float nx, fx, ny, fy, nz, fz;  // (org - o) inv moved out, for entry and exit
+    uint32_t qnx, qfx, qny, qfy, qnz, qfz;  // byte words of the entry and exit bounds
+    RTG_HD Node4Test(const Bvh4Node &n, V3 o, V3 inv) {
+        const uint32_t e = __builtin_bit_cast(uint32_t, n.org.w);
+        sx = pow2f(e & 0xFF) * inv.x, sy = pow2f((e >> 8) & 0xFF) * inv.y, sz = pow2f((e >> 16) & 0xFF) * inv.z;
+        const float bx = (n.org.x - o.x) * inv.x, by = (n.org.y - o.y) * inv.y, bz = (n.org.z - o.z) * inv.z;
+        const float mx = fabsf(bx) * 9.5367431640625e-07f + fabsf(sx) * 1.220703125e-04f;  // 2^-20, 2^-13
+        const float my = fabsf(by) * 9.5367431640625e-07f + fabsf(sy) * 1.220703125e-04f;
+        const float mz = fabsf(bz) * 9.5367431640625e-07f + fabsf(sz) * 1.220703125e-04f;
+        nx = bx - mx, fx = bx + mx, ny = by - my, fy = by + my, nz = bz - mz, fz = bz + mz;
+        // a negative direction enters a box through its upper bound
+        const bool neg_x = inv.x < 0.0f, neg_y = inv.y < 0.0f, neg_z = inv.z < 0.0f;
+        qnx = neg_x ? n.qxy.y : n.qxy.x, qfx = neg_x ? n.qxy.x : n.qxy.y;
+        qny = neg_y ? n.qxy.w : n.qxy.z, qfy = neg_y ? n.qxy.z : n.qxy.w;
+        qnz = neg_z ? n.qz.y : n.qz.x, qfz = neg_z ? n.qz.x : n.qz.y;
+    }
+    // Child k reaches the segment [0, tmax]; `near` gets its entry distance.
+    RTG_HD bool child(int k, float tmax, float &near) const {
+        const int sh = 8 * k;
+        const float tnx = fmaf((float)((qnx >> sh) & 0xFF), sx, nx);
+        const float tfx = fmaf((float)((qfx >> sh) & 0xFF), sx, fx);
+        const float tny = fmaf((float)((qny >> sh) & 0xFF), sy, ny);
+        const float tfy = fmaf((float)((qfy >> sh) & 0xFF), sy, fy);
+        const float tnz = fmaf((float)((qnz >> sh) & 0xFF), sz, nz);
+        const float tfz = fmaf((float)((qfz >> sh) & 0xFF), sz, fz);
+        near = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.0f));
+        const float far = fminf(fminf(tfx, tfy), tfz) * kFarScale;
+        return near <= fminf(far, tmax);
+    }
+};
+
 // Host emulation stack (the device uses an LDS-backed stack, rtg_kernels.hip).
 struct ArrayStack {
     int d[128];
@@ -101,9 +146,6 @@ RTG_HD void bvh_walk(const SceneView &S, const Ray &r, float &tmax, Stack &st, V
     st.reset();
     if (S.n_prims == 0) return;
     const V3 inv = safe_inv(r.d);
-    // per axis, whether the direction is negative: then the entry distance
-    // of a box on that axis comes from its upper bound
-    const bool nx = inv.x < 0.0f, ny = inv.y < 0.0f, nz = inv.z < 0.0f;
     int node = S.root;
     // One 4-wide node: the next node to enter (the nearest child the segment
     // reaches), the other children it reaches pushed farthest first; the
@@ -115,29 +157,9 @@ RTG_HD void bvh_walk(const SceneView &S, const Ray &r, float &tmax, Stack &st, V
         const int lk[4] = {n.link.x, n.link.y, n.link.z, n.link.w};
         float t[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         bool h[4];
-        // The slab test of `slab` on each child's decoded box, with the entry
-        // and exit bound of each axis picked by the direction's sign instead
-        // of ordering the two distances: the same values, since the distances
-        // keep the bounds' order when inv > 0 and reverse it when inv < 0.
-        const uint32_t e = __builtin_bit_cast(uint32_t, n.org.w);
-        const float sx = pow2f(e & 0xFF), sy = pow2f((e >> 8) & 0xFF), sz = pow2f((e >> 16) & 0xFF);
-        const uint32_t enx = nx ? n.qxy.y : n.qxy.x, fx = nx ? n.qxy.x : n.qxy.y;
-        const uint32_t eny = ny ? n.qxy.w : n.qxy.z, fy = ny ? n.qxy.z : n.qxy.w;
-        const uint32_t enz = nz ? n.qz.y : n.qz.x, fz = nz ? n.qz.x : n.qz.y;
+        const Node4Test nt(n, r.o, inv);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int sh = 8 * k;
-            const float ax = (dequant(n.org.x, (enx >> sh) & 0xFF, sx) - r.o.x) * inv.x;
-            const float bx = (dequant(n.org.x, (fx >> sh) & 0xFF, sx) - r.o.x) * inv.x;
-            const float ay = (dequant(n.org.y, (eny >> sh) & 0xFF, sy) - r.o.y) * inv.y;
-            const float by = (dequant(n.org.y, (fy >> sh) & 0xFF, sy) - r.o.y) * inv.y;
-            const float az = (dequant(n.org.z, (enz >> sh) & 0xFF, sz) - r.o.z) * inv.z;
-            const float bz = (dequant(n.org.z, (fz >> sh) & 0xFF, sz) - r.o.z) * inv.z;
-            const float near = fmaxf(fmaxf(ax, ay), fmaxf(az, 0.0f));
-            const float far = fminf(fminf(bx, by), bz) * kFarScale;
-            t[k] = near;
-            h[k] = k < nc && near <= fminf(far, tmax);
-        }
+        for (int k = 0; k < 4; ++k) h[k] = k < nc && nt.child(k, tmax, t[k]);
         int next = kWalkDone;
         if constexpr (kOrdered) {
             // Sort keys: the entry distance's bits (t >= 0, so they order as

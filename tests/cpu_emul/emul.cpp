@@ -392,4 +392,30 @@ int emul_node4(const float *boxes, const int32_t *links, int nc, float *decoded,
     return m;
 }
 
+// The walk's child test (Node4Test) against `slab` on the decoded boxes, for
+// one ray and the node `emul_node4` would make: per stored child k, out[4k..]
+// = {Node4Test passes, its entry distance, slab passes, slab's entry}.
+int emul_child_test(const float *boxes, const int32_t *links, int nc, const float *o3, const float *d3, float tmax,
+                    float *out) {
+    if (nc < 2 || nc > 4) return -1;
+    Box6 b[4];
+    int l[4];
+    for (int c = 0; c < nc; ++c) {
+        for (int a = 0; a < 3; ++a) b[c].lo[a] = boxes[6 * c + a], b[c].hi[a] = boxes[6 * c + 3 + a];
+        l[c] = links[c];
+    }
+    const Bvh4Node n = ordered_node4(b, l, nc);
+    const V3 o{o3[0], o3[1], o3[2]}, inv = safe_inv(V3{d3[0], d3[1], d3[2]});
+    const Node4Test nt(n, o, inv);
+    for (int k = 0; k < node4_count(n); ++k) {
+        float t1 = 0, t2 = 0, lx, hx, ly, hy, lz, hz;
+        out[4 * k] = nt.child(k, tmax, t1) ? 1.0f : 0.0f;
+        out[4 * k + 1] = t1;
+        node4_box(n, k, lx, hx, ly, hy, lz, hz);
+        out[4 * k + 2] = slab(o, inv, lx, hx, ly, hy, lz, hz, tmax, t2) ? 1.0f : 0.0f;
+        out[4 * k + 3] = t2;
+    }
+    return node4_count(n);
+}
+
 }  // extern "C"

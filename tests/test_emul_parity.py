@@ -329,3 +329,53 @@ def test_top_levels_are_laid_out_first():
         assert perm_ok == 1 and top_ok == 1, list(out)
         if n4 >= 31:
             assert n_top == 31  # five full binary levels
+
+
+def test_child_test_never_stricter_than_the_slab():
+    """The walk tests a node's children with one fused multiply-add per bound
+    and an outward margin (rt_trace.hpp Node4Test).  It must keep every child
+    the robust slab test keeps on the decoded box, with an entry distance no
+    larger, or it could cull a hit.  Random nodes and rays: origins inside
+    and outside the node, directions with zero components (+0 and -0),
+    near-parallel, 30 decades of scale, finite and infinite segment ends."""
+    import ctypes as C
+    from conftest import emul_lib
+    L = emul_lib()
+    L.emul_child_test.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_float, C.c_void_p]
+    rng = np.random.default_rng(33)
+    out = np.zeros(16, np.float32)
+    kept = 0
+    for trial in range(40000):
+        nc = int(rng.integers(2, 5))
+        scale = np.float32(10.0 ** rng.uniform(-15, 15))
+        centre = (rng.standard_normal(3) * scale * rng.choice([0.0, 1.0, 50.0])).astype(np.float32)
+        lo = (centre + rng.standard_normal((nc, 3)).astype(np.float32) * scale).astype(np.float32)
+        hi = (lo + np.abs(rng.standard_normal((nc, 3))).astype(np.float32) * scale * rng.choice([0.0, 1.0])).astype(
+            np.float32)
+        boxes = np.ascontiguousarray(np.concatenate([lo, np.maximum(hi, lo)], axis=1).astype(np.float32))
+        links = -np.arange(1, nc + 1, dtype=np.int32)
+        o = (centre + rng.standard_normal(3) * scale * rng.choice([0.5, 3.0, 30.0])).astype(np.float32)
+        target = boxes[rng.integers(nc), :3] + (boxes[rng.integers(nc), 3:] - boxes[rng.integers(nc), :3]) * rng.random(3)
+        d = (target - o + rng.standard_normal(3) * scale * rng.choice([0.0, 0.01, 1.0])).astype(np.float64)
+        kind = trial % 5
+        if kind == 1:
+            d[rng.integers(3)] = 0.0
+        elif kind == 2:
+            d[rng.integers(3)] = -0.0
+        elif kind == 3:
+            d[rng.integers(3)] *= 1e-7
+        nrm = np.linalg.norm(d)
+        if not np.isfinite(nrm) or nrm == 0:
+            continue
+        d = (d / nrm).astype(np.float32)
+        tmax = np.float32(np.inf) if trial % 3 == 0 else np.float32(np.linalg.norm(target - o) * rng.uniform(0.5, 2.0))
+        m = L.emul_child_test(boxes.ctypes.data, links.ctypes.data, nc, o.ctypes.data, d.ctypes.data, tmax,
+                              out.ctypes.data)
+        assert m == nc
+        for k in range(m):
+            fast, t_fast, exact, t_exact = out[4 * k:4 * k + 4]
+            if exact:
+                kept += 1
+                assert fast, (trial, k, boxes, o, d, tmax, out[4 * k:4 * k + 4])
+                assert t_fast <= t_exact, (trial, k, t_fast, t_exact)
+    assert kept > 3000
