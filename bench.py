@@ -67,8 +67,10 @@ def traffic_model(tr) -> dict:
     }
 
 
-# Launch tuning of the 1080p frame, chosen by rtg_autotune on MI355X.
-FRAME_TUNING = dict(tile_w=8, grab=1, grid_pct=75)
+# Launch tuning of the 1080p frame, chosen by rtg_autotune on MI355X: 16x4
+# pixel blocks per wave, one 64-item chunk per grab, persistent grids at half
+# the resident blocks (the render and side streams' kernels run side by side).
+FRAME_TUNING = dict(tile_w=16, grab=1, grid_pct=50)
 
 
 def pmc_traffic(engine: str, kernel: str):
