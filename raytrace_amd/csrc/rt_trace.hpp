@@ -158,8 +158,11 @@ RTG_HD void bvh_walk(const SceneView &S, const Ray &r, float &tmax, Stack &st, V
         float t[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         bool h[4];
         const Node4Test nt(n, r.o, inv);
+        // a node has 2 to 4 children; all four are tested (the bytes of a
+        // missing child decode to a point) and the count masks the result,
+        // which keeps the wave's four tests free of branches
 #pragma unroll
-        for (int k = 0; k < 4; ++k) h[k] = k < nc && nt.child(k, tmax, t[k]);
+        for (int k = 0; k < 4; ++k) h[k] = nt.child(k, tmax, t[k]) & (k < nc);
         int next = kWalkDone;
         if constexpr (kOrdered) {
             // Sort keys: the entry distance's bits (t >= 0, so they order as
