@@ -18,6 +18,7 @@ struct ScenePrep {
     std::vector<DevPart> parts;
     std::vector<float4> part_mid;  // model-space border midpoints
     std::vector<int> tri_part;
+    std::vector<int> tri_mtl;  // the triangle's part's material
     std::vector<const rtg_triangle *> part_tris;  // per global part, NULL for hidden models
     std::vector<int> part_order;                  // parts in global triangle order
     std::vector<DevMaterial> mtls;
@@ -79,6 +80,7 @@ inline std::string prepare_scene(const rtg_scene_desc &d, ScenePrep &P) {
             first_prim += p.tri_count;
             P.n_tris += p.tri_count;
             P.tri_part.insert(P.tri_part.end(), (size_t)p.tri_count, pi);
+            P.tri_mtl.insert(P.tri_mtl.end(), (size_t)p.tri_count, p.mtl);
         }
     }
     if (P.n_tris + (int64_t)P.spheres.size() >= (1 << 28)) return "more than 2^28 primitives";

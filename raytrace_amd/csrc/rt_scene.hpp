@@ -12,8 +12,9 @@
 //                        clTri values p0 + position, p1 - p0, p2 - p0
 //              sphere:   (centre, ~obj) (radius^2, radius, 0, 0) (0)
 //   tri_*    per-triangle shading data in scene order, read only for the
-//            winning hit: 3 normals, 3 uv, part index, octant-bin mask; and
-//            the model-space vertices, read only by the FP64 near-tie re-test.
+//            winning hit: 3 normals, 3 uv, part index, material index,
+//            octant-bin mask; and the model-space vertices, read only by the
+//            FP64 near-tie re-test.
 //   objects  Scene::Objects order; planes, cubes and ballplanes are tested
 //            outside the BVH through `brute`.
 #pragma once
@@ -130,6 +131,7 @@ struct SceneView {
     const float2 *tri_uv;   // 3 per triangle
     const float4 *tri_pos;  // 3 per triangle: model-space vertices (FP64 near-tie re-test only)
     const int *tri_part;
+    const int *tri_mtl;  // material of the triangle's part (read beside tri_part)
     const uint8_t *tri_bins;
     const DevMaterial *mtls;
     const DevLight *lights;

@@ -42,10 +42,12 @@ RTG_HD HitRec make_hitrec(const SceneView &S, const TraceRay &q, const Hit &h) {
         hr.N = normalize((n0 * h.w + n1 * h.u) + n2 * h.v);
         hr.tu = (uv[0].x * h.w + uv[1].x * h.u) + uv[2].x * h.v;
         hr.tv = (uv[0].y * h.w + uv[1].y * h.u) + uv[2].y * h.v;
-        const DevPart &P = S.parts[S.tri_part[g]];
-        hr.mtl = P.mtl;
-        hr.rfr = S.mtls[P.mtl].rfr;
-        hr.tex = S.mtls[P.mtl].tex;
+        // the material comes from its own per-triangle table, so that its
+        // load does not wait for the part's
+        const int mi = S.tri_mtl[g];
+        hr.mtl = mi;
+        hr.rfr = S.mtls[mi].rfr;
+        hr.tex = S.mtls[mi].tex;
         hr.id.bin = record_bin(S, q.r, q.src, g);
         break;
     }

@@ -371,6 +371,7 @@ static int scene_update(rtg_scene_t scene, const rtg_scene_desc *desc, bool allo
     take(R.nodes, fresh.nodes), take(R.prims, fresh.prims), take(R.objs, fresh.objs), take(R.brute, fresh.brute);
     take(R.parts, fresh.parts), take(R.tri_nrm, fresh.tri_nrm), take(R.tri_uv, fresh.tri_uv);
     take(R.tri_pos, fresh.tri_pos), take(R.tri_part, fresh.tri_part), take(R.tri_bins, fresh.tri_bins);
+    take(R.tri_mtl, fresh.tri_mtl);
     take(R.mtls, fresh.mtls), take(R.lights, fresh.lights), take(R.texs, fresh.texs), take(R.texels, fresh.texels);
     take(R.t_keys, fresh.t_keys), take(R.t_left, fresh.t_left), take(R.t_right, fresh.t_right);
     take(R.t_first, fresh.t_first), take(R.t_last, fresh.t_last), take(R.t_pint, fresh.t_pint);

@@ -229,7 +229,7 @@ int widen_tree(const DevBuf &nodes2, int n2, SceneRes &S, hipStream_t stream, in
 // Device bytes a resident scene holds: render data plus the kept topology.
 static size_t scene_bytes(const SceneRes &S) {
     return S.nodes.bytes + S.prims.bytes + S.objs.bytes + S.brute.bytes + S.parts.bytes + S.tri_nrm.bytes +
-           S.tri_uv.bytes + S.tri_pos.bytes + S.tri_part.bytes + S.tri_bins.bytes + S.mtls.bytes + S.lights.bytes +
+           S.tri_uv.bytes + S.tri_pos.bytes + S.tri_part.bytes + S.tri_mtl.bytes + S.tri_bins.bytes + S.mtls.bytes + S.lights.bytes +
            S.texs.bytes + S.texels.bytes + S.t_keys.bytes + S.t_left.bytes + S.t_right.bytes + S.t_first.bytes +
            S.t_last.bytes + S.t_pint.bytes + S.t_pleaf.bytes + S.t_newidx.bytes;
 }
@@ -277,7 +277,8 @@ int scene_build(SceneRes &S, const rtg_scene_desc &d, hipStream_t stream, SceneR
     if ((st = upload(S.objs, objs, stream)) || (st = upload(S.brute, brute, stream)) ||
         (st = upload(S.parts, parts, stream)) || (st = upload(S.mtls, mtls, stream)) ||
         (st = upload(S.lights, lights, stream)) || (st = upload(S.texs, texs, stream)) ||
-        (st = upload(S.texels, texels, stream)) || (st = upload(S.tri_part, tri_part, stream)))
+        (st = upload(S.texels, texels, stream)) || (st = upload(S.tri_part, tri_part, stream)) ||
+        (st = upload(S.tri_mtl, prep.tri_mtl, stream)))
         return st;
     const int n = (int)(n_tris + (int64_t)spheres.size());
     // refit: the donor's tree has the same primitive count (and the caller
@@ -431,6 +432,7 @@ int scene_build(SceneRes &S, const rtg_scene_desc &d, hipStream_t stream, SceneR
     v.tri_uv = S.tri_uv.as<float2>();
     v.tri_pos = S.tri_pos.as<float4>();
     v.tri_part = S.tri_part.as<int>();
+    v.tri_mtl = S.tri_mtl.as<int>();
     v.tri_bins = S.tri_bins.as<uint8_t>();
     v.mtls = S.mtls.as<DevMaterial>();
     v.lights = S.lights.as<DevLight>();

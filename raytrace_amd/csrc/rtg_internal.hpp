@@ -39,7 +39,7 @@ struct DevBuf {
 
 struct SceneRes {
     int device = 0;
-    DevBuf nodes, prims, objs, brute, parts, tri_nrm, tri_uv, tri_pos, tri_part, tri_bins, mtls, lights, texs, texels;
+    DevBuf nodes, prims, objs, brute, parts, tri_nrm, tri_uv, tri_pos, tri_part, tri_mtl, tri_bins, mtls, lights, texs, texels;
     SceneView view{};
     rtg_camera camera{};
     uint64_t geom_hash = 0;

@@ -195,6 +195,7 @@ int build(const rtg_scene_desc &d, HostScene &H) {
     v.tri_uv = H.tri_uv.data();
     v.tri_pos = H.tri_pos.data();
     v.tri_part = P.tri_part.data();
+    v.tri_mtl = P.tri_mtl.data();
     v.tri_bins = H.tri_bins.data();
     v.mtls = P.mtls.data();
     v.lights = P.lights.data();
