@@ -580,13 +580,17 @@ static int blocks_per_cu(K kernel) {
 
 template <bool B>
 struct Grids {
-    int extend, shade, occlude;
-    // pct: percent of the resident blocks (at least one block per CU)
+    int extend, shade, occlude, occlude_alone;
+    // pct: percent of the resident blocks (at least one block per CU).
+    // k_shade takes half that share: its waves mostly wait on their appends
+    // and stores, and fewer of them finish sooner (measured).  Shadow rays
+    // that run beside no traversal of a next level (the last level's, or all
+    // of them on one stream) take every resident block.
     explicit Grids(int cus, int pct = 100) {
         static const int oe = blocks_per_cu(k_extend<B, false>), os = blocks_per_cu(k_shade<B>),
                          oo = blocks_per_cu(k_occlude<B>);
-        auto part = [&](int o) { return std::max(cus, cus * o * pct / 100); };
-        extend = part(oe), shade = part(os), occlude = part(oo);
+        auto part = [&](int o, int p) { return std::max(cus, cus * o * p / 100); };
+        extend = part(oe, pct), shade = part(os, pct / 2), occlude = part(oo, pct), occlude_alone = part(oo, 100);
     }
 };
 
@@ -652,7 +656,7 @@ static int reserve_level(Workspace &ws, const WavePlan &pl, int k, size_t n_k) {
 static int reserve_all(Workspace &ws, const WavePlan &pl, int cus) {
     const Grids<false> g0(cus);
     const Grids<true> g1(cus);
-    const int gmax = std::max(std::max(g0.extend, g0.occlude), std::max(g1.extend, g1.occlude));
+    const int gmax = std::max(std::max(g0.extend, g0.occlude_alone), std::max(g1.extend, g1.occlude_alone));
     if (ws.grow(ws.spill, spill_ints(gmax) * sizeof(int))) return fail(RTG_E_NOMEM, "spill");
     if (ws.grow(ws.spill2, spill_ints(std::max(g0.occlude, g1.occlude)) * sizeof(int)))
         return fail(RTG_E_NOMEM, "spill");
@@ -675,7 +679,7 @@ static int enqueue_wave(const SceneView &S0, const WaveArgs &a, const WavePlan &
     const int cus = A.cus > 0 ? A.cus : 256;
     const Grids<B> g(cus, A.tune.grid_pct);
     if (!pl.no_sync) {
-        if (ws.grow(ws.spill, spill_ints(std::max(g.extend, g.occlude)) * sizeof(int)))
+        if (ws.grow(ws.spill, spill_ints(std::max(g.extend, g.occlude_alone)) * sizeof(int)))
             return fail(RTG_E_NOMEM, "spill");
         if (ws.grow(ws.w_qcount, (size_t)kPassCtrWords * sizeof(unsigned)))
             return fail(RTG_E_NOMEM, "queue counters");
@@ -755,7 +759,7 @@ static int enqueue_wave(const SceneView &S0, const WaveArgs &a, const WavePlan &
         }
         if (pl.n_on > 0) {
             kt = ktime_begin(A, sk, KT_OCCLUDE, 0);
-            k_occlude<B><<<grid(g.occlude, n_k * pl.n_on), kBlockThreads, 0, sk>>>(
+            k_occlude<B><<<grid(side_k ? g.occlude : g.occlude_alone, n_k * pl.n_on), kBlockThreads, 0, sk>>>(
                 S0, side_k ? oargs : args, ws.w_nodes[k].as<WNode>(), ws.w_shadow[k].as<unsigned>(), ctr + k,
                 ws.w_masks[k].as<unsigned>());
             ktime_end(A, sk, kt, &ctr[k].n_shadow.v);
