@@ -307,6 +307,46 @@ __global__ __launch_bounds__(kBlockThreads, 6) void k_extend(SceneView S0, WaveA
     }
 }
 
+// k_shade's appends are staged per wave in LDS and reach their queue a buffer
+// at a time, one atomic each: returning atomics on one queue counter
+// serialise across the chip (about 11 ns apart on MI355X), and two per 64
+// items bounded the kernel.
+constexpr int kStageRays = 160;    // child rays (48 B each)
+constexpr int kStageShadow = 512;  // shadow-ray entries
+
+// Orders a wave's LDS writes before its later LDS reads by other lanes (the
+// LDS executes one wave's accesses in order; this stops the compiler moving
+// them).
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Exclusive prefix of `count` over the wave; `total` (wave-uniform) gets the sum.
+__device__ inline int wave_prefix(int count, int &total) {
+    const int lane = __lane_id();
+    int x = count;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    total = __builtin_amdgcn_readfirstlane(__shfl(x, 63, 64));
+    return x - count;
+}
+
+// Move the wave's n staged items (n wave-uniform) to the end of a queue.
+template <class T>
+__device__ inline void flush_stage(const T *stage, int n, unsigned *counter, T *queue) {
+    if (n == 0) return;
+    int b = 0;
+    if (__lane_id() == 0) b = (int)atomicAdd(counter, (unsigned)n);
+    b = __builtin_amdgcn_readfirstlane(b);
+    wave_lds_sync();
+    for (int j = __lane_id(); j < n; j += 64) queue[b + j] = stage[j];
+    wave_lds_sync();  // before the stage is written again
+}
+
 template <bool B>
 __global__ __launch_bounds__(kBlockThreads, 4) void k_shade(SceneView S0, WaveArgs A, int k, const WRay *rays,
                                                          const unsigned *n_ptr, const WHit *hits, const int4 *cs,
@@ -317,6 +357,11 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_shade(SceneView S0, WaveAr
     if (wave_cancelled(A)) return;
     WaveQueue wq(ctr->sched[1], n, A.grab_lg);
     uint32_t c_shadow = 0;
+    __shared__ __align__(16) WRay stage_rays[kBlockThreads / 64][kStageRays];
+    __shared__ unsigned stage_shadow[kBlockThreads / 64][kStageShadow];
+    WRay *const sr = stage_rays[threadIdx.x >> 6];
+    unsigned *const ss = stage_shadow[threadIdx.x >> 6];
+    int n_sr = 0, n_ss = 0;  // staged items (wave-uniform)
     for (int base = wq.next(); base >= 0; base = wq.next()) {
         const int i = base + __lane_id();
         int n_child = 0;
@@ -409,9 +454,15 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_shade(SceneView S0, WaveAr
             slots[2 * i] = make_float4(0.0f, 0.0f, 0.0f, kMiss);
             slots[2 * i + 1] = make_float4(0.0f, 0.0f, 0.0f, kMiss);
         }
-        const int cb = wave_append(&ctr->n_next.v, n_child);
-        if (n_child > 0) next[cb] = c0;
-        if (n_child > 1) next[cb + 1] = c1;
+        int t_child;
+        const int pc = wave_prefix(n_child, t_child);
+        if (n_sr + t_child > kStageRays) {
+            flush_stage(sr, n_sr, &ctr->n_next.v, next);
+            n_sr = 0;
+        }
+        if (n_child > 0) sr[n_sr + pc] = c0;
+        if (n_child > 1) sr[n_sr + pc + 1] = c1;
+        n_sr += t_child;
         // Shadow entries (node << 5 | light), light-major within the wave's
         // block, so that a wave of k_occlude traces neighbouring points toward
         // one light.  Lights that cannot change the colour are not traced
@@ -425,17 +476,35 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_shade(SceneView S0, WaveAr
         const uint32_t any = B ? wave_or_u32(emit) : S0.lights_on;
         int total = 0;
         for (uint32_t lm = any; lm; lm &= lm - 1) total += __popcll(__ballot((emit >> __builtin_ctz(lm)) & 1u));
-        int sb = 0;
-        if (__lane_id() == 0 && total > 0) sb = (int)atomicAdd(&ctr->n_shadow.v, (unsigned)total);
-        sb = __shfl(sb, 0, 64);
+        total = __builtin_amdgcn_readfirstlane(total);
         const unsigned long long below = (1ull << __lane_id()) - 1ull;
-        for (uint32_t lm = any; lm; lm &= lm - 1) {
-            const int li = __builtin_ctz(lm);
-            const unsigned long long m = __ballot((emit >> li) & 1u);
-            if ((emit >> li) & 1u) shadow[sb + __popcll(m & below)] = ((unsigned)i << 5) | (unsigned)li;
-            sb += __popcll(m);
+        if (total > kStageShadow) {  // many lights: straight to the queue
+            int sb = 0;
+            if (__lane_id() == 0) sb = (int)atomicAdd(&ctr->n_shadow.v, (unsigned)total);
+            sb = __builtin_amdgcn_readfirstlane(sb);
+            for (uint32_t lm = any; lm; lm &= lm - 1) {
+                const int li = __builtin_ctz(lm);
+                const unsigned long long m = __ballot((emit >> li) & 1u);
+                if ((emit >> li) & 1u) shadow[sb + __popcll(m & below)] = ((unsigned)i << 5) | (unsigned)li;
+                sb += __popcll(m);
+            }
+        } else {
+            if (n_ss + total > kStageShadow) {
+                flush_stage(ss, n_ss, &ctr->n_shadow.v, shadow);
+                n_ss = 0;
+            }
+            int sb = n_ss;
+            for (uint32_t lm = any; lm; lm &= lm - 1) {
+                const int li = __builtin_ctz(lm);
+                const unsigned long long m = __ballot((emit >> li) & 1u);
+                if ((emit >> li) & 1u) ss[sb + __popcll(m & below)] = ((unsigned)i << 5) | (unsigned)li;
+                sb += __popcll(m);
+            }
+            n_ss += total;
         }
     }
+    flush_stage(sr, n_sr, &ctr->n_next.v, next);
+    flush_stage(ss, n_ss, &ctr->n_shadow.v, shadow);
     add_counts(A.counts, 0, c_shadow, 0, 0);
 }
 
