@@ -221,6 +221,17 @@ __device__ inline bool surplus_block(int n) {
     return (int)blockIdx.x * (kBlockThreads / 64) >= ((n + 63) >> 6);
 }
 
+// Grab size (64 << lg items) for n items: the tuning's, doubled up to eight
+// chunks while every wave of the grid still gets four grabs or more.  Each
+// grab is a returning atomic on its XCD's counter, a round trip of about 2 us
+// when every wave contends; small levels keep single chunks, so that their
+// items still spread over the grid.
+__device__ inline int grab_lg_for(int n, int lg) {
+    const int waves = (int)gridDim.x * (kBlockThreads / 64), chunks = (n + 63) >> 6;
+    while (lg < 3 && (chunks >> (lg + 1)) >= 4 * waves) ++lg;
+    return lg;
+}
+
 // R: some request asks for the FP64 near-tie re-test (a separate kernel, so
 // the plain closest-hit walk keeps its registers).
 template <bool B, bool R>
@@ -355,7 +366,7 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_shade(SceneView S0, WaveAr
                                                          unsigned *shadow, float4 *slots) {
     const int n = k == 0 ? A.n_samples : (int)*n_ptr;
     if (wave_cancelled(A)) return;
-    WaveQueue wq(ctr->sched[1], n, A.grab_lg);
+    WaveQueue wq(ctr->sched[1], n, grab_lg_for(n, A.grab_lg));
     uint32_t c_shadow = 0;
     __shared__ __align__(16) WRay stage_rays[kBlockThreads / 64][kStageRays];
     __shared__ unsigned stage_shadow[kBlockThreads / 64][kStageShadow];
