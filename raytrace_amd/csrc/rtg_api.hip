@@ -147,6 +147,7 @@ int Workspace::side_init() {
     for (int k = 0; k < kLevels; ++k)
         if (ev_fork[k] == nullptr) RTG_HIP(hipEventCreateWithFlags(&ev_fork[k], hipEventDisableTiming));
     if (ev_join == nullptr) RTG_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    if (ev_join2 == nullptr) RTG_HIP(hipEventCreateWithFlags(&ev_join2, hipEventDisableTiming));
     return RTG_OK;
 }
 
@@ -158,9 +159,12 @@ void Workspace::release() {
     }
     if (ev_join) (void)hipEventDestroy(ev_join);
     ev_join = nullptr;
+    if (ev_join2) (void)hipEventDestroy(ev_join2);
+    ev_join2 = nullptr;
     if (side) (void)hipStreamDestroy(side);
     side = nullptr;
     spill2.release();
+    spill3.release();
     if (graph) (void)hipGraphExecDestroy(graph);
     graph = nullptr;
     if (done) (void)hipEventDestroy(done);

@@ -91,8 +91,8 @@ struct Workspace {
     // shadow rays run on the render stream beside it (own spill area: two
     // traversal kernels at once).  Created on first use, default priority.
     hipStream_t side = nullptr;
-    hipEvent_t ev_fork[kLevels] = {}, ev_join = nullptr;
-    DevBuf spill2;
+    hipEvent_t ev_fork[kLevels] = {}, ev_join = nullptr, ev_join2 = nullptr;
+    DevBuf spill2, spill3;  // spill areas of the side stream and of the second side stream
     int side_init();
     // Grow `b` to hold `bytes`; bumps the epoch if it moved.
     int grow(DevBuf &b, size_t bytes);

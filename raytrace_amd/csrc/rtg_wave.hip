@@ -776,11 +776,27 @@ static int enqueue_wave(const SceneView &S0, const WaveArgs &a, const WavePlan &
         if (ws.grow(ws.spill2, spill_ints(g.occlude) * sizeof(int))) return fail(RTG_E_NOMEM, "spill");
     }
     hipStream_t so = ov ? ws.side : stream;
-    bool forked = false;  // the side stream took work (the join waits for it)
+    // A large single frame on a caller's stream, all levels enqueued at once,
+    // alternates the shadow rays of its levels k < max between the side
+    // stream and the arena's own stream (idle meanwhile: the caller holds the
+    // arena lock), so that level k's need not wait for level k - 1's; each
+    // stream has its own spill area.  Smaller frames, and frames that read
+    // level counts back, were slower with it (measured): their shadow rays
+    // do not bound the frame.
+    const bool ov2 = !B && ov && pl.no_sync && pl.n_samples >= (1 << 20) && A.stream != nullptr &&
+                     A.stream != stream && A.stream != ws.side;
+    if (ov2 && ws.grow(ws.spill3, spill_ints(g.occlude) * sizeof(int))) return fail(RTG_E_NOMEM, "spill");
+    hipStream_t so2 = ov2 ? A.stream : so;
+    bool forked = false, forked2 = false;  // a side stream took work (the join waits for it)
     auto join = [&]() -> int {
-        if (!forked) return RTG_OK;
-        RTG_HIP(hipEventRecord(ws.ev_join, so));
-        RTG_HIP(hipStreamWaitEvent(stream, ws.ev_join, 0));
+        if (forked) {
+            RTG_HIP(hipEventRecord(ws.ev_join, so));
+            RTG_HIP(hipStreamWaitEvent(stream, ws.ev_join, 0));
+        }
+        if (forked2) {
+            RTG_HIP(hipEventRecord(ws.ev_join2, so2));
+            RTG_HIP(hipStreamWaitEvent(stream, ws.ev_join2, 0));
+        }
         return RTG_OK;
     };
     LevelCtr *ctr = ws.w_qcount.as<LevelCtr>();
@@ -792,8 +808,9 @@ static int enqueue_wave(const SceneView &S0, const WaveArgs &a, const WavePlan &
     args.spill = ws.spill.as<int>();
     args.cancel = cancel ? cancel->dev : nullptr;
     args.abort = ws.w_qcount.as<unsigned>() + Workspace::kLevels * kCtrWords;
-    WaveArgs oargs = args;
+    WaveArgs oargs = args, oargs2 = args;
     if (ov) oargs.spill = ws.spill2.as<int>();
+    if (ov2) oargs2.spill = ws.spill3.as<int>();
     size_t n_k = (size_t)pl.n_samples;
     int last = -1;
     const size_t cap = level_ray_cap(pl.n_on);
@@ -830,18 +847,18 @@ static int enqueue_wave(const SceneView &S0, const WaveArgs &a, const WavePlan &
         // The last level has no next level to overlap: its shadow rays stay
         // on the render stream (with that stream's spill area), where they
         // run beside the previous level's shadow rays on the side stream.
-        const bool side_k = ov && k < pl.max_level;
-        hipStream_t sk = side_k ? so : stream;
+        const bool side_k = ov && k < pl.max_level, second = side_k && ov2 && (k & 1);
+        hipStream_t sk = side_k ? (second ? so2 : so) : stream;
         if (side_k) {
             RTG_HIP(hipEventRecord(ws.ev_fork[k], stream));
-            RTG_HIP(hipStreamWaitEvent(so, ws.ev_fork[k], 0));
-            forked = true;
+            RTG_HIP(hipStreamWaitEvent(sk, ws.ev_fork[k], 0));
+            (second ? forked2 : forked) = true;
         }
         if (pl.n_on > 0) {
             kt = ktime_begin(A, sk, KT_OCCLUDE, 0);
             k_occlude<B><<<grid(side_k ? g.occlude : g.occlude_alone, n_k * pl.n_on), kBlockThreads, 0, sk>>>(
-                S0, side_k ? oargs : args, ws.w_nodes[k].as<WNode>(), ws.w_shadow[k].as<unsigned>(), ctr + k,
-                ws.w_masks[k].as<unsigned>());
+                S0, side_k ? (second ? oargs2 : oargs) : args, ws.w_nodes[k].as<WNode>(),
+                ws.w_shadow[k].as<unsigned>(), ctr + k, ws.w_masks[k].as<unsigned>());
             ktime_end(A, sk, kt, &ctr[k].n_shadow.v);
         }
         RTG_HIP(hipGetLastError());
