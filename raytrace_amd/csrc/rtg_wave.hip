@@ -47,7 +47,8 @@ struct WNode {  // 64 B
     float px, py, pz, nx, ny, nz, dx, dy, dz, vr, vg, vb, t;
     int mtl_req;  // bits 0-19 material, 20-31 request
     int id_prim;
-    int bits;  // bits 0-23 object, 24-26 copy bin, 27 valid, 28 inside, 29 total internal reflection
+    int bits;  // bits 0-23 object, 24-26 copy bin, 27 valid, 28 inside, 29 total internal reflection,
+               // 30 a reflection child was traced, 31 a refraction child was traced
 };
 constexpr int kMtlBits = 20;
 constexpr int kMaxBatch = 1 << (32 - kMtlBits);
@@ -426,10 +427,11 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_shade(SceneView S0, WaveAr
                 const V3 vc = texel(S, hr.tex, hr.tu, hr.tv);
                 const DevMaterial &m = S.mtls[hr.mtl];
                 const float m_refract = R.no_refract ? 0.0f : m.refract;
-                int tir = 0;
+                int tir = 0, kids = 0;  // bit 0: reflection child, bit 1: refraction child
                 if (m.reflect > 0.01f && k + 1 <= R.max_level && bwc * m.reflect >= 1e-5f) {
                     c0 = make_wray(reflect_ray(q, hr), bwc * m.reflect, 1.0f, 2 * i, r);
                     n_child = 1;
+                    kids = 1;
                 }
                 if (m_refract > 0.01f) {
                     TraceRay cq;
@@ -442,6 +444,7 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_shade(SceneView S0, WaveAr
                         else
                             c1 = cw;
                         ++n_child;
+                        kids |= 2;
                     }
                 }
                 nd.px = hr.P.x, nd.py = hr.P.y, nd.pz = hr.P.z;
@@ -452,7 +455,7 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_shade(SceneView S0, WaveAr
                 nd.mtl_req = hr.mtl | (r << kMtlBits);
                 nd.id_prim = hr.id.prim;
                 nd.bits = (hr.id.obj & 0xFFFFFF) | ((hr.id.bin & 7) << 24) | (1 << 27) | ((hr.inside & 1) << 28) |
-                          (tir << 29);
+                          (tir << 29) | (int)((unsigned)kids << 30);
                 lit = true;
                 P = hr.P, N = hr.N, D = q.r.d;
                 on = S.lights_on;
@@ -460,10 +463,9 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_shade(SceneView S0, WaveAr
             }
             nodes[i] = nd;
             masks[i] = 0;
-            // the node's two child slots start empty (k_combine of the next
-            // level writes the children it has)
-            slots[2 * i] = make_float4(0.0f, 0.0f, 0.0f, kMiss);
-            slots[2 * i + 1] = make_float4(0.0f, 0.0f, 0.0f, kMiss);
+            // k_combine of the next level writes the child slot of every
+            // child traced (bits 30, 31); k_combine of this level reads only
+            // those, so the slots are not cleared here
         }
         int t_child;
         const int pc = wave_prefix(n_child, t_child);
@@ -574,7 +576,9 @@ __global__ __launch_bounds__(kBlockThreads, 8) void k_combine(SceneView S0, Wave
                                      mtl, V3{nd.vr, nd.vg, nd.vb},
                                      [&](int li, const LightTerm &, bool) -> bool { return (mask >> li) & 1u; });
             const DevMaterial &m = S.mtls[mtl];
-            const float4 a = slots[2 * i], b = slots[2 * i + 1];
+            const float4 none = make_float4(0.0f, 0.0f, 0.0f, kMiss);  // no child: black, no Beer factor
+            const float4 a = (nd.bits >> 30) & 1 ? slots[2 * i] : none;
+            const float4 b = (nd.bits >> 31) & 1 ? slots[2 * i + 1] : none;
             const V3 col = compose(L, m.reflect, R.no_refract ? 0.0f : m.refract, (nd.bits >> 29) & 1,
                                    (nd.bits >> 28) & 1, m.kd, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, b.w);
             out = make_float4(col.x, col.y, col.z, nd.t);
