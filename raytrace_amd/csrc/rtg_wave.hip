@@ -11,7 +11,8 @@
 //   combine local Blinn-Phong from the light mask, composed with the node's
 //           children in the reference's order; written into the parent's
 //           child slot (level 0: the sample colour)
-//   resolve average of the samples of a pixel, quantised to RGB8
+//   resolve average of the samples of a pixel, quantised to RGB8 (with one
+//           sample per pixel, level 0's combine writes the pixel itself)
 // The per-node arithmetic is the megakernel's (shade_local, compose), so the
 // two engines give bit-identical frames.
 //
@@ -560,7 +561,16 @@ __global__ __launch_bounds__(kBlockThreads, 8) void k_combine(SceneView S0, Wave
                                                            const unsigned *masks, const float4 *slots,
                                                            float4 *parent_slots, float4 *samples) {
     const int n = k == 0 ? A.n_samples : (int)*n_ptr;
-    if (wave_cancelled(A)) return;
+    // Level 0 writes the pixels of requests with one sample per pixel (the
+    // box filter of one sample is that sample, quantised), in place of
+    // k_resolve.  Like k_resolve it reads the pass's abort word, which no
+    // kernel sets any more once level 0 combines, so a pass a cancel stopped
+    // keeps all its pixels untouched.
+    if (k == 0) {
+        if (A.cancel != nullptr && *(volatile unsigned *)A.abort != 0) return;
+    } else if (wave_cancelled(A)) {
+        return;
+    }
     const ChunkRange cr = xcd_chunks((n + kBlockThreads - 1) / kBlockThreads);
     for (int c = cr.first; c < cr.end; c += cr.step) {
         const int i = c * kBlockThreads + threadIdx.x;
@@ -583,10 +593,21 @@ __global__ __launch_bounds__(kBlockThreads, 8) void k_combine(SceneView S0, Wave
                                    (nd.bits >> 28) & 1, m.kd, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, b.w);
             out = make_float4(col.x, col.y, col.z, nd.t);
         }
-        if (k == 0)
-            samples[i] = out;
-        else
+        if (k != 0) {
             parent_slots[rays[i].link] = out;
+            continue;
+        }
+        const int r = sample_req<B>(A, i);
+        const WaveReq &Rq = req_at<B>(A, r < 0 ? 0 : r);
+        if (Rq.spp != 1) {
+            samples[i] = out;
+            continue;
+        }
+        int px, py, si;
+        if (r >= 0 && sample_pixel(Rq, i - Rq.sample_base, px, py, si)) {
+            uint8_t *p = Rq.rgb + 3 * ((size_t)py * Rq.W + px);
+            p[0] = quantize(out.x), p[1] = quantize(out.y), p[2] = quantize(out.z);
+        }
     }
 }
 
@@ -597,6 +618,7 @@ __global__ void k_resolve(WaveArgs A, const float4 *samples) {
     // a pass that a cancel stopped part-way keeps its pixels untouched (127)
     if (A.cancel != nullptr && *(volatile unsigned *)A.abort != 0) return;
     const WaveReq &R = req_at<B>(A, blockIdx.y);
+    if (R.spp == 1) return;  // written by level 0's k_combine
     // pixels of the pass: block rows row0 .. row0 + n_samples / (bw 64 spp)
     const int lg = R.tile_lg, th_lg = 6 - lg;
     const int y0 = R.row0 << th_lg;
@@ -894,9 +916,11 @@ static int enqueue_wave(const SceneView &S0, const WaveArgs &a, const WavePlan &
             ws.w_samples.as<float4>());
         ktime_end(A, stream, kt, n_ptr);
     }
-    const int kt = ktime_begin(A, stream, KT_RESOLVE, 0);
-    k_resolve<B><<<dim3(B ? 256 : 1024, pl.n_reqs), 256, 0, stream>>>(args, ws.w_samples.as<float4>());
-    ktime_end(A, stream, kt);
+    if (B || a.r0.spp != 1) {  // a single frame at one sample per pixel is written by k_combine
+        const int kt = ktime_begin(A, stream, KT_RESOLVE, 0);
+        k_resolve<B><<<dim3(B ? 256 : 1024, pl.n_reqs), 256, 0, stream>>>(args, ws.w_samples.as<float4>());
+        ktime_end(A, stream, kt);
+    }
     RTG_HIP(hipGetLastError());
     return RTG_OK;
 }
