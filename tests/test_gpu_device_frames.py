@@ -73,3 +73,28 @@ def test_repeated_device_frames(rt_gpu):
         assert (hip.fetch(rgb, (270, 480, 3)) == ref2).all(), f"device frame {k} after the update differs"
     hip.free(rgb)
     hip.free(rgb_o)
+
+
+def test_large_device_frame_on_a_caller_stream(rt_gpu):
+    """A frame of 2^20 samples or more on a caller's stream, all levels
+    enqueued at once, alternates its levels' shadow rays between the side
+    stream and the arena's own stream: every frame must equal the render with
+    all kernels on one stream."""
+    rt = rt_gpu
+    hip = _Hip()
+    ds = rt.DeviceScene(rt.HostScene.workload(2))
+    W, H = 1280, 1024  # 1.31 M samples
+    ref = ds.render(rt.params(W, H, max_level=4, flags=rt.RTG_FLAG_ONE_STREAM), ids=False, tprim=False)["rgb"]
+    p = rt.params(W, H, max_level=4)
+    s = C.c_void_p()
+    assert hip.lib.hipStreamCreate(C.byref(s)) == 0
+    rgb = hip.alloc(W * H * 3)
+    lib = rt._capi.load()
+    log(f"case: workload 2 at {W}x{H}, depth 4, 3 device frames on a caller stream")
+    for k in range(3):
+        rc = lib.rtg_render_device(ds.h, C.byref(p), rgb, None, None, None, s)
+        assert rc == 0, rt._capi.last_error()
+        assert hip.lib.hipStreamSynchronize(s) == 0
+        assert (hip.fetch(rgb, (H, W, 3)) == ref).all(), f"device frame {k} differs from the one-stream render"
+    hip.free(rgb)
+    assert hip.lib.hipStreamDestroy(s) == 0
