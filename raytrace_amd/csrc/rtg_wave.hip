@@ -462,8 +462,14 @@ __global__ __launch_bounds__(kBlockThreads, 4) void k_shade(SceneView S0, WaveAr
                 on = S.lights_on;
                 c_shadow += __builtin_popcount(on);  // the reference traces one shadow ray per switched-on light
             }
-            nodes[i] = nd;
-            masks[i] = 0;
+            // a ray without a hit leaves only its node's bits (nothing reads more:
+            // its light mask and child slots neither)
+            if ((nd.bits >> 27) & 1) {
+                nodes[i] = nd;
+                masks[i] = 0;
+            } else {
+                nodes[i].bits = 0;
+            }
             // k_combine of the next level writes the child slot of every
             // child traced (bits 30, 31); k_combine of this level reads only
             // those, so the slots are not cleared here
@@ -575,9 +581,9 @@ __global__ __launch_bounds__(kBlockThreads, 8) void k_combine(SceneView S0, Wave
     for (int c = cr.first; c < cr.end; c += cr.step) {
         const int i = c * kBlockThreads + threadIdx.x;
         if (i >= n) continue;
-        const WNode nd = nodes[i];
         float4 out = make_float4(0.0f, 0.0f, 0.0f, kMiss);
-        if ((nd.bits >> 27) & 1) {
+        if ((nodes[i].bits >> 27) & 1) {  // a node without a hit holds only its bits
+            const WNode nd = nodes[i];
             const WaveReq &R = req_at<B>(A, node_req(nd));
             const SceneView &S = scene_at<B>(S0, A, R.scene);
             const int mtl = node_mtl(nd);
