@@ -635,7 +635,8 @@ int rtg_kernel_times(rtg_kernel_time *out, int max, int *n) {
     if (!arena()) return fail(RTG_E_STATE, "call rtg_init first");
     Arena &A = *arena();
     std::lock_guard<std::mutex> l(A.lock);
-    static const char *names[KT_COUNT] = {"k_extend", "k_shade", "k_occlude", "k_combine", "k_resolve", "k_megakernel"};
+    static const char *names[KT_COUNT] = {"k_extend",  "k_shade",      "k_occlude", "k_combine",
+                                          "k_resolve", "k_megakernel", "k_local"};
     rtg_kernel_time acc[KT_COUNT];
     memset(acc, 0, sizeof(acc));
     int st = RTG_OK;

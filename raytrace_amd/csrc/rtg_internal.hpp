@@ -167,7 +167,7 @@ struct Arena {
     std::mutex lock;  // one render at a time on the arena
 };
 
-enum KernelKind { KT_EXTEND, KT_SHADE, KT_OCCLUDE, KT_COMBINE, KT_RESOLVE, KT_MEGA, KT_COUNT };
+enum KernelKind { KT_EXTEND, KT_SHADE, KT_OCCLUDE, KT_COMBINE, KT_RESOLVE, KT_MEGA, KT_LOCAL, KT_COUNT };
 // Start timing a launch of `kind` on `s` (no-op unless A.ktimes); returns a
 // record index for ktime_end, or -1.
 int ktime_begin(Arena &A, hipStream_t s, int kind, uint64_t items);

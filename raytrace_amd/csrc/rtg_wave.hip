@@ -562,10 +562,31 @@ __global__ __launch_bounds__(kBlockThreads, 8) void k_occlude(SceneView S0, Wave
 }
 
 template <bool B>
+__global__ __launch_bounds__(kBlockThreads, 8) void k_local(SceneView S0, WaveArgs A, int k, const unsigned *n_ptr,
+                                                         WNode *nodes, const unsigned *masks) {
+    const int n = k == 0 ? A.n_samples : (int)*n_ptr;
+    if (wave_cancelled(A)) return;
+    const ChunkRange cr = xcd_chunks((n + kBlockThreads - 1) / kBlockThreads);
+    for (int c = cr.first; c < cr.end; c += cr.step) {
+        const int i = c * kBlockThreads + threadIdx.x;
+        if (i >= n || !((nodes[i].bits >> 27) & 1)) continue;
+        const WNode nd = nodes[i];
+        const WaveReq &R = req_at<B>(A, node_req(nd));
+        const SceneView &S = scene_at<B>(S0, A, R.scene);
+        const unsigned mask = masks[i];
+        const V3 L = shade_local(S, V3{nd.dx, nd.dy, nd.dz}, V3{nd.px, nd.py, nd.pz}, V3{nd.nx, nd.ny, nd.nz},
+                                 node_mtl(nd), V3{nd.vr, nd.vg, nd.vb},
+                                 [&](int li, const LightTerm &, bool) -> bool { return (mask >> li) & 1u; });
+        nodes[i].vr = L.x, nodes[i].vg = L.y, nodes[i].vb = L.z;
+    }
+}
+
+// local: the node's colour fields hold its local lighting already (k_local).
+template <bool B>
 __global__ __launch_bounds__(kBlockThreads, 8) void k_combine(SceneView S0, WaveArgs A, int k, const WRay *rays,
                                                            const unsigned *n_ptr, const WNode *nodes,
                                                            const unsigned *masks, const float4 *slots,
-                                                           float4 *parent_slots, float4 *samples) {
+                                                           float4 *parent_slots, float4 *samples, int local) {
     const int n = k == 0 ? A.n_samples : (int)*n_ptr;
     // Level 0 writes the pixels of requests with one sample per pixel (the
     // box filter of one sample is that sample, quantised), in place of
@@ -587,10 +608,12 @@ __global__ __launch_bounds__(kBlockThreads, 8) void k_combine(SceneView S0, Wave
             const WaveReq &R = req_at<B>(A, node_req(nd));
             const SceneView &S = scene_at<B>(S0, A, R.scene);
             const int mtl = node_mtl(nd);
-            const unsigned mask = masks[i];
-            const V3 L = shade_local(S, V3{nd.dx, nd.dy, nd.dz}, V3{nd.px, nd.py, nd.pz}, V3{nd.nx, nd.ny, nd.nz},
-                                     mtl, V3{nd.vr, nd.vg, nd.vb},
-                                     [&](int li, const LightTerm &, bool) -> bool { return (mask >> li) & 1u; });
+            V3 L = V3{nd.vr, nd.vg, nd.vb};
+            if (!local) {
+                const unsigned mask = masks[i];
+                L = shade_local(S, V3{nd.dx, nd.dy, nd.dz}, V3{nd.px, nd.py, nd.pz}, V3{nd.nx, nd.ny, nd.nz}, mtl,
+                                L, [&](int li, const LightTerm &, bool) -> bool { return (mask >> li) & 1u; });
+            }
             const DevMaterial &m = S.mtls[mtl];
             const float4 none = make_float4(0.0f, 0.0f, 0.0f, kMiss);  // no child: black, no Beer factor
             const float4 a = (nd.bits >> 30) & 1 ? slots[2 * i] : none;
@@ -833,6 +856,8 @@ static int enqueue_wave(const SceneView &S0, const WaveArgs &a, const WavePlan &
     };
     LevelCtr *ctr = ws.w_qcount.as<LevelCtr>();
     k_zero_u32<<<(kPassCtrWords + 255) / 256, 256, 0, stream>>>(ws.w_qcount.as<unsigned>(), kPassCtrWords);
+    const int lblocks = std::max(8, cus * 16);  // k_local, k_combine
+    const bool early_local = !B && pl.no_sync;
     auto grid = [](int gmax, size_t items) {
         return (int)std::max<size_t>(8, std::min<size_t>(gmax, (items + 127) / 128));
     };
@@ -893,6 +918,18 @@ static int enqueue_wave(const SceneView &S0, const WaveArgs &a, const WavePlan &
                 ws.w_shadow[k].as<unsigned>(), ctr + k, ws.w_masks[k].as<unsigned>());
             ktime_end(A, sk, kt, &ctr[k].n_shadow.v);
         }
+        // Local lighting of every level but the last, once its shadow rays are
+        // done, on their stream (beside the next levels): the level's
+        // k_combine, on the frame's last stretch, then only composes.  Single
+        // frames with every level enqueued at once only: where levels wait for
+        // host read-backs (many lights, deep trees) the extra side work ended
+        // up on the frame's path (measured).
+        if (early_local && k < pl.max_level) {
+            kt = ktime_begin(A, sk, KT_LOCAL, 0);
+            k_local<B><<<lblocks, kBlockThreads, 0, sk>>>(S0, args, k, n_ptr, ws.w_nodes[k].as<WNode>(),
+                                                          ws.w_masks[k].as<unsigned>());
+            ktime_end(A, sk, kt, n_ptr);
+        }
         RTG_HIP(hipGetLastError());
         last = k;
         if (k == pl.max_level) break;
@@ -919,7 +956,7 @@ static int enqueue_wave(const SceneView &S0, const WaveArgs &a, const WavePlan &
         k_combine<B><<<blocks, kBlockThreads, 0, stream>>>(
             S0, args, k, rays, n_ptr, ws.w_nodes[k].as<WNode>(), ws.w_masks[k].as<unsigned>(),
             ws.w_slots[k].as<float4>(), k > 0 ? ws.w_slots[k - 1].as<float4>() : nullptr,
-            ws.w_samples.as<float4>());
+            ws.w_samples.as<float4>(), early_local && k < pl.max_level ? 1 : 0);
         ktime_end(A, stream, kt, n_ptr);
     }
     if (B || a.r0.spp != 1) {  // a single frame at one sample per pixel is written by k_combine
