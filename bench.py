@@ -35,10 +35,10 @@ def ray_bytes(rays, node_visits, tri_tests, sph_tests) -> float:
     return 64.0 * rays + 64.0 * node_visits + 48.0 * tri_tests + 16.0 * sph_tests
 
 
-def fp64_error(g: dict, o64: dict, W: int, H: int, tiles) -> dict:
-    """Pixel-error statistics of a device frame against the FP64 oracle on the
-    rendered tiles: primary-id mismatch fraction, RGB mean / max absolute error
-    (1/255 steps) and the fraction of pixels off by more than one step."""
+def fp64_counts(g: dict, o64: dict, W: int, H: int, tiles) -> list:
+    """[pixels, id mismatches, sum of RGB absolute errors, max RGB error, pixels
+    off by more than one step] of a device frame against the FP64 oracle on the
+    rendered tiles."""
     import numpy as np
     m = np.zeros((H, W), bool)
     tx = (W + 63) // 64
@@ -47,9 +47,53 @@ def fp64_error(g: dict, o64: dict, W: int, H: int, tiles) -> dict:
         m[by * 64:by * 64 + 64, bx * 64:bx * 64 + 64] = True
     ids_eq = (g["ids"] == o64["ids"]).all(-1)
     d = np.abs(g["rgb"].astype(np.int16) - o64["rgb"].astype(np.int16))
-    return {"pixels": int(m.sum()), "ids_mismatch": round(float((~ids_eq & m).sum() / m.sum()), 6),
-            "rgb_mae": round(float(d[m].mean()), 4), "rgb_max": int(d[m].max()),
-            "rgb_over_1": round(float(((d.max(-1) > 1) & m).sum() / m.sum()), 6)}
+    return [int(m.sum()), int((~ids_eq & m).sum()), float(d[m].astype(np.float64).sum()), int(d[m].max()),
+            int(((d.max(-1) > 1) & m).sum())]
+
+
+def fp64_stats(c: list) -> dict:
+    """fp64_counts (or their sums over frames) as the reported statistics:
+    primary-id mismatch fraction, RGB mean / max absolute error (1/255 steps)
+    and the fraction of pixels off by more than one step."""
+    px = max(1, c[0])
+    return {"pixels": c[0], "ids_mismatch": round(c[1] / px, 6), "rgb_mae": round(c[2] / (3 * px), 4),
+            "rgb_max": c[3], "rgb_over_1": round(c[4] / px, 6)}
+
+
+def fp64_error(g: dict, o64: dict, W: int, H: int, tiles) -> dict:
+    """Pixel-error statistics of a device frame against the FP64 oracle on the
+    rendered tiles."""
+    return fp64_stats(fp64_counts(g, o64, W, H, tiles))
+
+
+def request_errors(rt, hs, ds, reqs, got, ref, tiles_per: int = 4, seed: int = 7) -> dict:
+    """Pixel errors of the batch and serving workloads: every request's RGB in
+    `got` (batched) against `ref` (the same request rendered alone), byte by
+    byte, and the first request of each scene, rendered alone with its ids,
+    against the FP64 oracle on seeded 64x64 tiles of its frame."""
+    import numpy as np
+    from oracle import pyoracle as po
+    same = sum(int(np.array_equal(a, b)) for a, b in zip(got, ref))
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    rng = np.random.default_rng(seed)
+    tot = [0, 0, 0.0, 0, 0]
+    sampled = []
+    for s in range(len(hs)):
+        ks = [k for k, r in enumerate(reqs) if r[0] == s]
+        if not ks:
+            continue
+        _, w, h = reqs[ks[0]][:3]
+        p = rt.params(w, h, max_level=hs[s].info.max_level)
+        tx, ty = (w + 63) // 64, (h + 63) // 64
+        tiles = np.sort(rng.choice(tx * ty, size=min(tiles_per, tx * ty), replace=False)).astype(np.int32)
+        o64 = po.OracleScene(hs[s].desc_ptr).render(p, po.ORACLE_FP64, threads=threads, tiles=tiles)
+        c = fp64_counts(ds[s].render(p), o64, w, h, tiles)
+        tot = [tot[0] + c[0], tot[1] + c[1], tot[2] + c[2], max(tot[3], c[3]), tot[4] + c[4]]
+        sampled.append(f"{w}x{h}")
+    out = fp64_stats(tot)
+    out["sample"] = (f"the first request of each scene ({', '.join(sampled)}), up to {tiles_per} seeded "
+                     "64x64 tiles each, rendered alone")
+    return {"fp64_error": out, "requests_identical_to_alone": f"{same} of {len(got)}"}
 
 
 def traffic_model(tr) -> dict:
@@ -183,6 +227,7 @@ def bench_batch(args) -> dict:
         prays += d.render(p, ids=False, tprim=False, out=o)["stats"].rays_total
     plain_s = time.perf_counter() - t0
     assert prays == rays
+    errs = request_errors(rt, hs, ds, reqs, [o["rgb"] for o in outs], [o["rgb"] for o in pouts])
     return {
         "metric": "Mrays/s (primary+secondary), 256 batched requests", "value": round(rays / batch_s / 1e6, 2),
         "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -199,6 +244,7 @@ def bench_batch(args) -> dict:
         "plain": {"ms_per_batch": round(plain_s * 1e3, 3), "mrays_per_s": round(rays / plain_s / 1e6, 2),
                   "what": "the same 256 requests, one rtg_render each"},
         "speedup_vs_plain": round(plain_s / batch_s, 2),
+        **errs,
     }
 
 
@@ -218,13 +264,15 @@ def bench_serve(args) -> dict:
     for mode in (rt._capi.RTG_SERVE_BATCHED, rt._capi.RTG_SERVE_PLAIN):
         rt.serve(warm_items, mode=mode, slots=args.slots)
     rows = []
+    top_rgb = {}  # each mode's results at the top rate, for the pixel errors
     for rate in rates:
         n = max(8, int(rate * args.duration))
         reqs = rt.gen_requests(n, rate, seed=4)
         items = [(ds[s], rt.params(w, h, max_level=hs[s].info.max_level), t) for s, w, h, t in reqs]
         row = {"rate": rate, "requests": n}
         for name, mode in (("batched", rt._capi.RTG_SERVE_BATCHED), ("plain", rt._capi.RTG_SERVE_PLAIN)):
-            _, _, st = rt.serve(items, mode=mode, slots=args.slots)
+            res, _, st = rt.serve(items, mode=mode, slots=args.slots)
+            top_rgb[name] = (reqs, res) if rate == rates[-1] else None
             row[name] = {"p50_ms": round(st.p50_ms, 3), "p99_ms": round(st.p99_ms, 3),
                          "req_per_s": round(st.req_per_s, 1), "mrays_per_s": round(st.mrays_per_s, 1),
                          "batches": st.batches, "max_in_flight": st.max_in_flight}
@@ -251,6 +299,9 @@ def bench_serve(args) -> dict:
     best_p = max((r["plain"] for r in rows), key=lambda b: b["req_per_s"])
     ok_b = [r["batched"] for r in rows + cap_rows if r["batched"]["p99_ms"] <= best_p["p99_ms"]]
     best_b = max(ok_b or [top["batched"]], key=lambda b: b["req_per_s"])
+    # the batched loop's results against the plain renderer's (each request
+    # alone) at the top rate, byte by byte, and sampled requests against FP64
+    errs = request_errors(rt, hs, ds, top_rgb["batched"][0], top_rgb["batched"][1], top_rgb["plain"][1])
     return {
         "metric": "serve-loop p50/p99 frame latency and throughput", "value": top["batched"]["req_per_s"],
         "unit": "requests/s at the highest offered rate of the sweep", "n_gpus": 1, "steps": len(rates), "warmup": 0,
@@ -266,6 +317,7 @@ def bench_serve(args) -> dict:
         "throughput_vs_plain_at_top_rate": round(top["batched"]["req_per_s"] / max(1e-9, top["plain"]["req_per_s"]), 2),
         "max_throughput": {"plain": best_p, "batched_at_no_worse_p99": best_b,
                            "ratio": round(best_b["req_per_s"] / max(1e-9, best_p["req_per_s"]), 2)},
+        **errs,
     }
 
 

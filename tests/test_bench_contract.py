@@ -50,3 +50,30 @@ def test_traffic_scales_with_rays():
     many = bench.traffic_model(_traffic(rays=1000, node_visits=10000, tri_tests=4000))
     assert one["per_closest_ray"] == pytest.approx(many["per_closest_ray"])
     assert one["per_shadow_ray"] == 0.0  # no shadow rays: nothing divided by zero
+
+
+def test_pixel_error_statistics_count_only_the_sampled_tiles():
+    import numpy as np
+    # a 130 x 100 frame: 3 x 2 tiles of 64 (the last column and row partial);
+    # tiles 0 and 5 (the bottom-right corner, 2 x 36 pixels) are sampled
+    W, H = 130, 100
+    g = {"ids": np.zeros((H, W, 2), np.int32), "rgb": np.full((H, W, 3), 100, np.uint8)}
+    o = {"ids": g["ids"].copy(), "rgb": g["rgb"].copy()}
+    o["ids"][3, 5, 1] = 7        # tile 0: one id mismatch
+    o["rgb"][10, 10] = (103, 100, 100)  # tile 0: off by 3 in one channel
+    o["rgb"][99, 129] = (101, 99, 100)  # tile 5: off by 1 in two channels
+    o["ids"][70, 10] = (1, 1)    # tile 3: not sampled
+    o["rgb"][70, 10] = (0, 0, 0)
+    c = bench.fp64_counts(g, o, W, H, [0, 5])
+    px = 64 * 64 + 2 * 36
+    assert c == [px, 1, 5.0, 3, 1]
+    st = bench.fp64_stats(c)
+    assert st["pixels"] == px
+    assert st["ids_mismatch"] == round(1 / px, 6)
+    assert st["rgb_mae"] == round(5 / (3 * px), 4)
+    assert st["rgb_max"] == 3 and st["rgb_over_1"] == round(1 / px, 6)
+    # the one-frame statistics are these counts
+    assert bench.fp64_error(g, o, W, H, [0, 5]) == st
+    # counts of two frames add (the maximum takes the larger)
+    two = [a + b if k != 3 else max(a, b) for k, (a, b) in enumerate(zip(c, c))]
+    assert bench.fp64_stats(two)["ids_mismatch"] == st["ids_mismatch"] and bench.fp64_stats(two)["pixels"] == 2 * px
