@@ -66,34 +66,65 @@ def fp64_error(g: dict, o64: dict, W: int, H: int, tiles) -> dict:
     return fp64_stats(fp64_counts(g, o64, W, H, tiles))
 
 
-def request_errors(rt, hs, ds, reqs, got, ref, tiles_per: int = 4, seed: int = 7) -> dict:
-    """Pixel errors of the batch and serving workloads: every request's RGB in
-    `got` (batched) against `ref` (the same request rendered alone), byte by
-    byte, and the first request of each scene, rendered alone with its ids,
-    against the FP64 oracle on seeded 64x64 tiles of its frame."""
+def sample_requests(hs, reqs, tiles_per: int = 4, seed: int = 7) -> list:
+    """[(scene, width, height, tiles)]: the first request of each scene in
+    `reqs` and up to `tiles_per` seeded 64x64 tiles of its frame."""
     import numpy as np
-    from oracle import pyoracle as po
-    same = sum(int(np.array_equal(a, b)) for a, b in zip(got, ref))
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
     rng = np.random.default_rng(seed)
-    tot = [0, 0, 0.0, 0, 0]
-    sampled = []
+    out = []
     for s in range(len(hs)):
         ks = [k for k, r in enumerate(reqs) if r[0] == s]
         if not ks:
             continue
         _, w, h = reqs[ks[0]][:3]
-        p = rt.params(w, h, max_level=hs[s].info.max_level)
         tx, ty = (w + 63) // 64, (h + 63) // 64
         tiles = np.sort(rng.choice(tx * ty, size=min(tiles_per, tx * ty), replace=False)).astype(np.int32)
+        out.append((s, w, h, tiles))
+    return out
+
+
+def _sample_what(sample, tiles_per: int = 4) -> str:
+    return (f"the first request of each scene ({', '.join(f'{w}x{h}' for _, w, h, _ in sample)}), up to "
+            f"{tiles_per} seeded 64x64 tiles each")
+
+
+def request_errors(rt, hs, ds, reqs, got, ref) -> dict:
+    """Pixel errors of the batch and serving workloads: every request's RGB in
+    `got` (batched) against `ref` (the same request rendered alone), byte by
+    byte, and the sampled requests (sample_requests), rendered alone with their
+    ids, against the FP64 oracle on their sampled tiles."""
+    import numpy as np
+    from oracle import pyoracle as po
+    same = sum(int(np.array_equal(a, b)) for a, b in zip(got, ref))
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    sample = sample_requests(hs, reqs)
+    tot = [0, 0, 0.0, 0, 0]
+    for s, w, h, tiles in sample:
+        p = rt.params(w, h, max_level=hs[s].info.max_level)
         o64 = po.OracleScene(hs[s].desc_ptr).render(p, po.ORACLE_FP64, threads=threads, tiles=tiles)
         c = fp64_counts(ds[s].render(p), o64, w, h, tiles)
         tot = [tot[0] + c[0], tot[1] + c[1], tot[2] + c[2], max(tot[3], c[3]), tot[4] + c[4]]
-        sampled.append(f"{w}x{h}")
     out = fp64_stats(tot)
-    out["sample"] = (f"the first request of each scene ({', '.join(sampled)}), up to {tiles_per} seeded "
-                     "64x64 tiles each, rendered alone")
+    out["sample"] = _sample_what(sample) + ", rendered alone"
     return {"fp64_error": out, "requests_identical_to_alone": f"{same} of {len(got)}"}
+
+
+def cpu_requests(rt, hs, reqs) -> dict:
+    """The reference algorithm's port (the REF32 oracle) on the host cores over
+    the sampled tiles of the sampled requests (sample_requests), one wall-clock
+    run per request: Mrays/s."""
+    from oracle import pyoracle as po
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    sample = sample_requests(hs, reqs)
+    rays, secs, px = 0, 0.0, 0
+    for s, w, h, tiles in sample:
+        p = rt.params(w, h, max_level=hs[s].info.max_level)
+        st = po.OracleScene(hs[s].desc_ptr).render(p, po.ORACLE_REF32, threads=threads, tiles=tiles,
+                                                    want=("rgb",))["stats"]
+        rays, secs, px = rays + st.rays(), secs + st.seconds, px + st.pixels
+    return {"value": round(rays / secs / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"{_sample_what(sample)} ({px} pixels, {rays} reference rays, {secs:.1f} s); the rate "
+                      "extrapolates to the whole workload"}
 
 
 def traffic_model(tr) -> dict:
@@ -228,6 +259,7 @@ def bench_batch(args) -> dict:
     plain_s = time.perf_counter() - t0
     assert prays == rays
     errs = request_errors(rt, hs, ds, reqs, [o["rgb"] for o in outs], [o["rgb"] for o in pouts])
+    cpu = None if args.no_cpu_baseline else cpu_requests(rt, hs, reqs)
     return {
         "metric": "Mrays/s (primary+secondary), 256 batched requests", "value": round(rays / batch_s / 1e6, 2),
         "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -244,6 +276,7 @@ def bench_batch(args) -> dict:
         "plain": {"ms_per_batch": round(plain_s * 1e3, 3), "mrays_per_s": round(rays / plain_s / 1e6, 2),
                   "what": "the same 256 requests, one rtg_render each"},
         "speedup_vs_plain": round(plain_s / batch_s, 2),
+        "cpu_baseline": cpu,
         **errs,
     }
 
@@ -302,6 +335,13 @@ def bench_serve(args) -> dict:
     # the batched loop's results against the plain renderer's (each request
     # alone) at the top rate, byte by byte, and sampled requests against FP64
     errs = request_errors(rt, hs, ds, top_rgb["batched"][0], top_rgb["batched"][1], top_rgb["plain"][1])
+    cpu = None
+    if not args.no_cpu_baseline:
+        # the port's ray rate as requests/s of this workload (its rays per request)
+        c = cpu_requests(rt, hs, top_rgb["batched"][0])
+        per_req = top["batched"]["mrays_per_s"] * 1e6 / max(1e-9, top["batched"]["req_per_s"])
+        cpu = {**c, "value": round(c["value"] * 1e6 / per_req, 3), "unit": "requests/s",
+               "mrays_per_s": c["value"], "rays_per_request": round(per_req)}
     return {
         "metric": "serve-loop p50/p99 frame latency and throughput", "value": top["batched"]["req_per_s"],
         "unit": "requests/s at the highest offered rate of the sweep", "n_gpus": 1, "steps": len(rates), "warmup": 0,
@@ -317,6 +357,7 @@ def bench_serve(args) -> dict:
         "throughput_vs_plain_at_top_rate": round(top["batched"]["req_per_s"] / max(1e-9, top["plain"]["req_per_s"]), 2),
         "max_throughput": {"plain": best_p, "batched_at_no_worse_p99": best_b,
                            "ratio": round(best_b["req_per_s"] / max(1e-9, best_p["req_per_s"]), 2)},
+        "cpu_baseline": cpu,
         **errs,
     }
 
