@@ -426,15 +426,7 @@ def bench_stress(args) -> dict:
     threads = max(1, min(16, len(os.sched_getaffinity(0))))
     log(f"FP64 oracle on {len(tiles)} tiles")
     o64 = osc.render(p, po.ORACLE_FP64, threads=threads, tiles=tiles)
-    m = np.zeros((H, W), bool)
-    for t in tiles:
-        bx, by = t % tx, t // tx
-        m[by * 64:by * 64 + 64, bx * 64:bx * 64 + 64] = True
-    ids_eq = (out["ids"] == o64["ids"]).all(-1)
-    d = np.abs(out["rgb"].astype(np.int16) - o64["rgb"].astype(np.int16))
-    fp64 = {"pixels": int(m.sum()), "ids_mismatch": round(float((~ids_eq & m).sum() / m.sum()), 6),
-            "rgb_mae": round(float(d[m].mean()), 4), "rgb_max": int(d[m].max()),
-            "rgb_over_1": round(float(((d.max(-1) > 1) & m).sum() / m.sum()), 6)}
+    fp64 = fp64_error(out, o64, W, H, tiles)
     ref_tiles = tiles[:: max(1, len(tiles) // 3)][:3]
     log(f"reference-algorithm oracle on {len(ref_tiles)} tiles")
     o32 = osc.render(p, po.ORACLE_REF32, threads=threads, tiles=ref_tiles)
@@ -447,6 +439,11 @@ def bench_stress(args) -> dict:
     dr = np.abs(out["rgb"].astype(np.int16) - o32["rgb"].astype(np.int16)).max(-1)
     parity = {"pixels": int(mr.sum()), "ids_match": round(float(eq[mr].mean()), 6),
               "ids_mismatch_outside_band": int((~eq & ~band & mr).sum()), "rgb_over_1": int(((dr > 1) & eq & mr).sum())}
+    # the reference algorithm's port on the host cores: that same render, timed
+    s32 = o32["stats"]
+    cpu = {"value": round(s32.rays() / s32.seconds / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+           "sample": f"the {len(ref_tiles)} 64x64 tiles of the parity check ({s32.pixels} pixels, {s32.rays()} "
+                     f"reference rays, {s32.seconds:.1f} s), one wall-clock run; the rate extrapolates to the frame"}
     rays_traced = rays - per_type[1] + traced_shadow
     return {
         "metric": "Mrays/s (primary+secondary), 4K stress frame", "value": round(rays / wall_s / 1e6, 2),
@@ -462,7 +459,7 @@ def bench_stress(args) -> dict:
                    "rays_by_type": {"primary": st.rays_primary, "shadow": st.rays_shadow, "reflect": st.rays_reflect,
                                     "refract": st.rays_refract},
                    "bvh_build_ms": round(ds.build_ms, 2), "scene_bytes": ds.nbytes, "triangles": int(hs.info.n_tris)},
-        "fp64_error": fp64, "parity_vs_reference_algorithm": parity,
+        "cpu_baseline": cpu, "fp64_error": fp64, "parity_vs_reference_algorithm": parity,
     }
 
 
