@@ -77,3 +77,17 @@ def test_pixel_error_statistics_count_only_the_sampled_tiles():
     # counts of two frames add (the maximum takes the larger)
     two = [a + b if k != 3 else max(a, b) for k, (a, b) in enumerate(zip(c, c))]
     assert bench.fp64_stats(two)["ids_mismatch"] == st["ids_mismatch"] and bench.fp64_stats(two)["pixels"] == 2 * px
+
+
+def test_request_sample_takes_the_first_request_of_each_scene():
+    # scenes 0..3; scene 2 has no request; request 4 is scene 0's second
+    reqs = [(1, 64, 64, 0.0), (0, 130, 100, 0.0), (3, 1024, 768, 0.0), (1, 200, 200, 0.0), (0, 64, 64, 0.0)]
+    s = bench.sample_requests([None] * 4, reqs, tiles_per=4, seed=7)
+    assert [(sc, w, h) for sc, w, h, _ in s] == [(0, 130, 100), (1, 64, 64), (3, 1024, 768)]
+    for sc, w, h, tiles in s:
+        n = ((w + 63) // 64) * ((h + 63) // 64)
+        assert len(tiles) == min(4, n) and len(set(tiles.tolist())) == len(tiles)
+        assert all(0 <= t < n for t in tiles) and list(tiles) == sorted(tiles)
+    # seeded: the same sample every time
+    s2 = bench.sample_requests([None] * 4, reqs, tiles_per=4, seed=7)
+    assert all((a[3] == b[3]).all() for a, b in zip(s, s2))
