@@ -122,7 +122,7 @@ def cpu_requests(rt, hs, reqs) -> dict:
         st = po.OracleScene(hs[s].desc_ptr).render(p, po.ORACLE_REF32, threads=threads, tiles=tiles,
                                                     want=("rgb",))["stats"]
         rays, secs, px = rays + st.rays(), secs + st.seconds, px + st.pixels
-    return {"value": round(rays / secs / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+    return {"value": round(rays / max(secs, 1e-9) / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"{_sample_what(sample)} ({px} pixels, {rays} reference rays, {secs:.1f} s); the rate "
                       "extrapolates to the whole workload"}
 
